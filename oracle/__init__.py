@@ -1,0 +1,339 @@
+"""CPU oracle for the dedispersion + single-pulse detection path.
+
+TEST INFRASTRUCTURE ONLY: this package is the checker.  It may be imported by
+``tests/``, by ``__graft_entry__.smoke()`` and by the CPU-baseline leg of
+``bench.py``; the product package ``mdsm_amd`` never imports it.
+
+The arithmetic lives in ``oracle.c`` (built to ``liboracle.so`` by
+``oracle/Makefile``); this module only marshals numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build() -> None:
+    """Compile liboracle.so from oracle.c (gcc + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        build()
+    return C.CDLL(_LIB_PATH)
+
+
+_lib = _load()
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i = C.c_int
+_i64 = C.c_int64
+_d = C.c_double
+_f = C.c_float
+_u64 = C.c_uint64
+_u32 = C.c_uint32
+
+
+def _sig(name, res, *args):
+    fn = getattr(_lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+_sig("oracle_philox4x32_10", None, _u32p, _u32p, _u32p)
+_sig("oracle_channel_bandpass", None, _u64, _i, _d, _d, _d, _d, _f32p, _f32p)
+_sig("oracle_generate", None, _u64, _i, _d, _d, _d, _i64, _i64, _f32p, _f32p, _i,
+     _f64p, _i64p, _i32p, _f32p, _f32p, _i64)
+_sig("oracle_delay_table", None, _d, _d, _i, _d, _d, _d, _i, _i32p)
+_sig("oracle_dm_shifts_ref", None, _f, _f, _i, _f32p)
+_sig("oracle_delay_table_ref", None, _f, _f, _i, _f, _f, _f, _i, _i32p)
+_sig("oracle_maxshift_ref", _u32, _f32p, _f32p, _i, _i, _f, _f, _f, _i)
+_sig("oracle_shared_shift_ref", _u32, _f, _f, _f, _i, _f, _i)
+_sig("oracle_calculate_nsamp_ref", _u32, _u32, _u32, _u32, _u32, _u64)
+_sig("oracle_dedisperse_f64", None, _f32p, _i64, _i, _i, _i, _i32p, _f64p)
+_sig("oracle_dedisperse_f32seq", None, _f32p, _i64, _i, _i, _i, _i32p, _f32p)
+_sig("oracle_dedisperse_f16", None, _u16p, _i64, _i, _i, _i, _i32p, _f32p)
+_sig("oracle_dedisperse_u8", None, _u8p, _i64, _i, _i, _i, _i32p, _f32p)
+_sig("oracle_f32_to_f16_array", None, _f32p, _u16p, _i64)
+_sig("oracle_f16_to_f32_array", None, _u16p, _f32p, _i64)
+_sig("oracle_channel_stats", None, _f32p, _i64, _i, _i64, _f64p, _f64p)
+_sig("oracle_normalise_f32", None, _f32p, _i64, _i, _i64, _f64p, _f64p, _f32p, _i64)
+_sig("oracle_normalise_f16", None, _f32p, _i64, _i, _i64, _f64p, _f64p, _u16p, _i64)
+_sig("oracle_normalise_u8", None, _f32p, _i64, _i, _i64, _f64p, _f64p, _u8p, _i64)
+_sig("oracle_block_stats", None, _f64p, _i, _i, _i, _f64p, _f64p)
+_sig("oracle_detect", _i64, _f64p, _i, _i, _i, _f64p, _f64p, _i, _d, _i64,
+     _i32p, _i32p, _i32p, _f32p)
+_sig("oracle_global_stats", None, _f32p, _i64, C.POINTER(_d), C.POINTER(_d))
+_sig("oracle_threshold_ref", _i64, _f32p, _i, _i, _d, _d, _f, _i64, _i32p, _i32p, _f32p)
+_sig("oracle_dbscan2d", _i, _i32p, _i32p, _i, _i, _i, _i32p)
+_sig("oracle_dbscan_ref", _i, _f64p, _f64p, _f64p, _u32, _f, _f, _f, _u32, _i32p)
+_sig("oracle_median5", _f, _f, _f, _f, _f, _f)
+_sig("oracle_median_filter5", None, _f32p, _i, _i, _f32p)
+_sig("oracle_process_brute_legacy", _i64, _f32p, _i, _i, _f, _f, _f, _i64, _i32p, _i32p,
+     _f32p, C.POINTER(_f), C.POINTER(_f))
+_sig("oracle_detrend_normalise", None, _f32p, _i64, _i, _f32p)
+_sig("oracle_num_threads", _i)
+
+KDM = 4148.808
+U8_SCALE = 8.0 / 255.0
+U8_OFFSET = -4.0
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, np.uint32)
+    _lib.oracle_philox4x32_10(_c(ctr, np.uint32), _c(key, np.uint32), out)
+    return out
+
+
+def channel_bandpass(seed, nchans, gain=(0.5, 2.0), offset=(0.0, 10.0)):
+    g = np.empty(nchans, np.float32)
+    o = np.empty(nchans, np.float32)
+    _lib.oracle_channel_bandpass(seed, nchans, gain[0], gain[1], offset[0], offset[1], g, o)
+    return g, o
+
+
+def generate(seed, nchans, fch1, foff, tsamp, t0, n, gain=None, offset=None, pulses=(), ld=None):
+    """Raw filterbank block [nchans][ld] (float32).  pulses: iterable of
+    (dm, start, width, amp) with amp in normalised units."""
+    ld = n if ld is None else ld
+    if gain is None:
+        gain = np.ones(nchans, np.float32)
+    if offset is None:
+        offset = np.zeros(nchans, np.float32)
+    p = list(pulses)
+    p_dm = np.array([q[0] for q in p] or [0.0], np.float64)
+    p_st = np.array([q[1] for q in p] or [0], np.int64)
+    p_w = np.array([q[2] for q in p] or [0], np.int32)
+    p_a = np.array([q[3] for q in p] or [0.0], np.float32)
+    out = np.zeros((nchans, ld), np.float32)
+    _lib.oracle_generate(seed, nchans, fch1, foff, tsamp, t0, n, _c(gain, np.float32),
+                         _c(offset, np.float32), len(p), p_dm, p_st, p_w, p_a, out, ld)
+    return out
+
+
+def delay_table(fch1, foff, nchans, tsamp, lowdm, dmstep, ndms):
+    t = np.empty((nchans, ndms), np.int32)
+    _lib.oracle_delay_table(fch1, foff, nchans, tsamp, lowdm, dmstep, ndms, t)
+    return t
+
+
+def dm_shifts_ref(fch1, foff, nchans):
+    s = np.empty(nchans, np.float32)
+    _lib.oracle_dm_shifts_ref(fch1, foff, nchans, s)
+    return s
+
+
+def delay_table_ref(fch1, foff, nchans, tsamp, lowdm, dmstep, ndms):
+    t = np.empty((nchans, ndms), np.int32)
+    _lib.oracle_delay_table_ref(fch1, foff, nchans, tsamp, lowdm, dmstep, ndms, t)
+    return t
+
+
+def maxshift_ref(fch1s, foffs, nchans, tsamp, lowdm, dmstep, ndms):
+    return int(_lib.oracle_maxshift_ref(_c(fch1s, np.float32), _c(foffs, np.float32), len(fch1s),
+                                        nchans, tsamp, lowdm, dmstep, ndms))
+
+
+def shared_shift_ref(bottom_shift, lowdm, dmstep, ndms, tsamp, dedisp_dms=32):
+    return int(_lib.oracle_shared_shift_ref(bottom_shift, lowdm, dmstep, ndms, tsamp, dedisp_dms))
+
+
+def calculate_nsamp_ref(nsamp, maxshift, nchans, ndms, memory_mb):
+    return int(_lib.oracle_calculate_nsamp_ref(nsamp, maxshift, nchans, ndms, memory_mb))
+
+
+def dedisperse_f64(x, table, nsamp):
+    x = _c(x, np.float32)
+    nchans, ld = x.shape
+    ndms = table.shape[1]
+    out = np.empty((ndms, nsamp), np.float64)
+    _lib.oracle_dedisperse_f64(x, ld, nchans, nsamp, ndms, _c(table, np.int32), out)
+    return out
+
+
+def dedisperse_f32seq(x, table, nsamp, out=None):
+    x = _c(x, np.float32)
+    nchans, ld = x.shape
+    ndms = table.shape[1]
+    if out is None:
+        out = np.empty((ndms, nsamp), np.float32)
+    _lib.oracle_dedisperse_f32seq(x, ld, nchans, nsamp, ndms, _c(table, np.int32), out)
+    return out
+
+
+def dedisperse_f16(h, table, nsamp):
+    h = _c(h, np.uint16)
+    nchans, ld = h.shape
+    ndms = table.shape[1]
+    out = np.empty((ndms, nsamp), np.float32)
+    _lib.oracle_dedisperse_f16(h, ld, nchans, nsamp, ndms, _c(table, np.int32), out)
+    return out
+
+
+def dedisperse_u8(u, table, nsamp):
+    u = _c(u, np.uint8)
+    nchans, ld = u.shape
+    ndms = table.shape[1]
+    out = np.empty((ndms, nsamp), np.float32)
+    _lib.oracle_dedisperse_u8(u, ld, nchans, nsamp, ndms, _c(table, np.int32), out)
+    return out
+
+
+def f32_to_f16(x):
+    x = _c(x, np.float32)
+    out = np.empty(x.shape, np.uint16)
+    _lib.oracle_f32_to_f16_array(x.reshape(-1), out.reshape(-1), x.size)
+    return out
+
+
+def f16_to_f32(h):
+    h = _c(h, np.uint16)
+    out = np.empty(h.shape, np.float32)
+    _lib.oracle_f16_to_f32_array(h.reshape(-1), out.reshape(-1), h.size)
+    return out
+
+
+def channel_stats(raw, n=None):
+    raw = _c(raw, np.float32)
+    nchans, ld = raw.shape
+    n = ld if n is None else n
+    m = np.empty(nchans, np.float64)
+    s = np.empty(nchans, np.float64)
+    _lib.oracle_channel_stats(raw, ld, nchans, n, m, s)
+    return m, s
+
+
+def normalise(raw, mean, std, dtype="f32", n=None):
+    raw = _c(raw, np.float32)
+    nchans, ld = raw.shape
+    n = ld if n is None else n
+    mean = _c(mean, np.float64)
+    std = _c(std, np.float64)
+    if dtype == "f32":
+        out = np.empty((nchans, n), np.float32)
+        _lib.oracle_normalise_f32(raw, ld, nchans, n, mean, std, out, n)
+    elif dtype == "f16":
+        out = np.empty((nchans, n), np.uint16)
+        _lib.oracle_normalise_f16(raw, ld, nchans, n, mean, std, out, n)
+    elif dtype == "u8":
+        out = np.empty((nchans, n), np.uint8)
+        _lib.oracle_normalise_u8(raw, ld, nchans, n, mean, std, out, n)
+    else:
+        raise ValueError(dtype)
+    return out
+
+
+def block_stats(x, L):
+    x = _c(x, np.float64)
+    nrows, nsamp = x.shape
+    nb = (nsamp + L - 1) // L
+    m = np.empty((nrows, nb), np.float64)
+    s = np.empty((nrows, nb), np.float64)
+    _lib.oracle_block_stats(x, nrows, nsamp, L, m, s)
+    return m, s
+
+
+def detect(x, L, nwidths, thresh, cap=1 << 20, stats=None):
+    """Returns (d, t, k, snr) arrays sorted by (d, t) and the total count."""
+    x = _c(x, np.float64)
+    nrows, nsamp = x.shape
+    m, s = block_stats(x, L) if stats is None else stats
+    cd = np.empty(cap, np.int32)
+    ct = np.empty(cap, np.int32)
+    ck = np.empty(cap, np.int32)
+    cs = np.empty(cap, np.float32)
+    n = _lib.oracle_detect(x, nrows, nsamp, L, _c(m, np.float64), _c(s, np.float64), nwidths,
+                           thresh, cap, cd, ct, ck, cs)
+    k = min(n, cap)
+    return cd[:k].copy(), ct[:k].copy(), ck[:k].copy(), cs[:k].copy(), int(n)
+
+
+def global_stats(x):
+    x = _c(x, np.float32)
+    m = C.c_double()
+    r = C.c_double()
+    _lib.oracle_global_stats(x.reshape(-1), x.size, C.byref(m), C.byref(r))
+    return m.value, r.value
+
+
+def threshold_ref(x, mean, rms, thresh, cap=1 << 20):
+    x = _c(x, np.float32)
+    ndms, nsamp = x.shape
+    cd = np.empty(cap, np.int32)
+    ct = np.empty(cap, np.int32)
+    cs = np.empty(cap, np.float32)
+    n = _lib.oracle_threshold_ref(x, ndms, nsamp, mean, rms, thresh, cap, cd, ct, cs)
+    k = min(n, cap)
+    return cd[:k].copy(), ct[:k].copy(), cs[:k].copy(), int(n)
+
+
+def dbscan2d(d, t, eps, minpts):
+    d = _c(d, np.int32)
+    t = _c(t, np.int32)
+    lab = np.empty(len(d), np.int32)
+    n = _lib.oracle_dbscan2d(d, t, len(d), eps, minpts, lab)
+    return lab, int(n)
+
+
+def dbscan_ref(time, dm, snr, min_time, min_dm, min_snr, min_points):
+    time = _c(time, np.float64)
+    dm = _c(dm, np.float64)
+    snr = _c(snr, np.float64)
+    lab = np.empty(len(time), np.int32)
+    n = _lib.oracle_dbscan_ref(time, dm, snr, len(time), min_time, min_dm, min_snr, min_points, lab)
+    return lab, int(n)
+
+
+def median5(a, b, c, d, e):
+    return float(_lib.oracle_median5(a, b, c, d, e))
+
+
+def median_filter5(x):
+    x = _c(x, np.float32)
+    out = np.empty_like(x)
+    _lib.oracle_median_filter5(x, x.shape[0], x.shape[1], out)
+    return out
+
+
+def process_brute_legacy(x, lowdm, dmstep, thresh, cap=1 << 20):
+    x = _c(x, np.float32)
+    ndms, nsamp = x.shape
+    cd = np.empty(cap, np.int32)
+    ct = np.empty(cap, np.int32)
+    cs = np.empty(cap, np.float32)
+    m = C.c_float()
+    r = C.c_float()
+    n = _lib.oracle_process_brute_legacy(x, ndms, nsamp, lowdm, dmstep, thresh, cap, cd, ct, cs,
+                                         C.byref(m), C.byref(r))
+    k = min(n, cap)
+    return cd[:k].copy(), ct[:k].copy(), cs[:k].copy(), int(n), m.value, r.value
+
+
+def detrend_normalise(x, L):
+    x = _c(x, np.float32).reshape(-1)
+    nseg = x.size // L
+    out = np.empty(nseg * L, np.float32)
+    _lib.oracle_detrend_normalise(x, nseg, L, out)
+    return out
+
+
+def num_threads():
+    return int(_lib.oracle_num_threads())

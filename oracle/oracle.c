@@ -1,0 +1,775 @@
+/*
+ * oracle.c -- CPU restatement of MDSM's brute-force dedispersion and
+ * single-pulse detection path.  TEST INFRASTRUCTURE ONLY.
+ *
+ * This file is the checker.  Only the tests, the smoke check and the CPU
+ * baseline leg of the benchmark may load it; the product library never links
+ * or calls it.  Everything here is plain C (gcc, OpenMP), written for clarity
+ * and for bit-exact reproducibility, not for speed (except
+ * oracle_dedisperse_f32seq, which doubles as the multithreaded CPU baseline).
+ *
+ * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).  Floating
+ * point contraction is disabled so that every expression rounds exactly as
+ * written; the product kernels are compiled the same way for the
+ * expressions that must match bit for bit (delay table, statistics, boxcar
+ * tree, thresholds).
+ *
+ * Reference functions restated (paths relative to the MDSM tree):
+ *   delay constant / dm_shifts  src/mdsm-multibeam/multibeam_dedispersion_manager.cpp:325-326,366-369
+ *   maxshift / nsamp            src/mdsm-multibeam/multibeam_dedispersion_manager.cpp:329-347,371-387
+ *   integer delay table         src/mdsm-multibeam/multibeam_dedispersion_thread.cu:546-559
+ *   LDS halo (shared_shift)     src/mdsm-multibeam/multibeam_dedispersion_thread.cu:542-543
+ *   brute-force sum             src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:44-95
+ *   global mean/rms + threshold src/mdsm-multibeam/multibeam_dedispersion_output.cpp:148-185,188-248
+ *   DBSCAN (3-D box variant)    src/mdsm-multibeam/dbscan.cpp:194-276,390-415
+ *   median of five (legacy)     src/mdsm/dedispersion_output.cpp:14-53,135-213
+ *   detrend + normalise         src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:207-295
+ * North-star extensions restated from their own definitions (no reference
+ * code exists for them): FP64 round() delay table with 4148.808, per-block
+ * FP64 mean/RMS, dyadic boxcar detection, 2-D Chebyshev DBSCAN, FP16/uint8
+ * input storage, Philox4x32-10 synthetic data.
+ *
+ * Pinning (what checks each function; see tests/test_oracle_*.py):
+ *   philox4x32_10      Random123 known-answer vectors (Salmon et al. 2011)
+ *   delay tables       closed-form special cases (DM=0, top channel, exact
+ *                      integer configurations), monotonicity, +-0.5 bound
+ *   dedisperse_*       known answers (in[c][t]=c, constant, impulse on a DM
+ *                      track, DM 0 = channel sum), FP64 vs FP32 agreement,
+ *                      numpy brute force on tiny inputs
+ *   stats              numpy mean/std (tolerance) and exact sums on
+ *                      integer-valued data (order independent)
+ *   boxcar detect      numpy cumulative-sum boxcar (tolerance), exact on
+ *                      integer data, injected pulse recovered at its DM/width
+ *   dbscan2d           scikit-learn DBSCAN(metric="chebyshev")
+ *   dbscan_ref         hand-derived cases from the reference algorithm's
+ *                      text (the reference needs Qt headers: unbuildable here)
+ *   median5            sorted median over random and exhaustive-order inputs
+ *   f16 conversion     numpy float16 (round to nearest even)
+ */
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define KDM_NORTHSTAR 4148.808
+#define KDM_REFERENCE 4148.741601
+#define R256 256
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based RNG */
+/* ------------------------------------------------------------------ */
+static inline void mulhilo32(uint32_t a, uint32_t b, uint32_t *hi, uint32_t *lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  *lo = (uint32_t)p;
+}
+
+void oracle_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo32(0xD2511F53u, c0, &hi0, &lo0);
+    mulhilo32(0xCD9E8D57u, c2, &hi1, &lo1);
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n1 = lo1;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    uint32_t n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline double u24(uint32_t x) { return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0); }
+
+/* Four N(0,1) draws for channel c, sample quad q (samples 4q..4q+3). */
+static void gauss4(uint64_t seed, uint32_t c, uint64_t q, float z[4]) {
+  uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), c, 0u};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t o[4];
+  oracle_philox4x32_10(ctr, key, o);
+  const double two_pi = 6.283185307179586;
+  double r0 = sqrt(-2.0 * log(u24(o[0]))), a0 = two_pi * u24(o[1]);
+  double r1 = sqrt(-2.0 * log(u24(o[2]))), a1 = two_pi * u24(o[3]);
+  z[0] = (float)(r0 * cos(a0));
+  z[1] = (float)(r0 * sin(a0));
+  z[2] = (float)(r1 * cos(a1));
+  z[3] = (float)(r1 * sin(a1));
+}
+
+/* Per-channel bandpass gain and offset, drawn from stream 1. */
+void oracle_channel_bandpass(uint64_t seed, int nchans, double g_lo, double g_hi, double o_lo,
+                             double o_hi, float *gain, float *offset) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int c = 0; c < nchans; ++c) {
+    uint32_t ctr[4] = {(uint32_t)c, 0u, 0u, 1u}, o[4];
+    oracle_philox4x32_10(ctr, key, o);
+    gain[c] = (float)(g_lo + (g_hi - g_lo) * u24(o[0]));
+    offset[c] = (float)(o_lo + (o_hi - o_lo) * u24(o[1]));
+  }
+}
+
+/* Delay of channel c for an arbitrary DM (north-star formula, FP64, round). */
+static inline int64_t ns_delay(double fch1, double foff, int c, double tsamp, double dm) {
+  double f = fch1 + foff * (double)c;
+  double k = 1.0 / (f * f) - 1.0 / (fch1 * fch1);
+  return (int64_t)round(((KDM_NORTHSTAR * dm) * k) / tsamp);
+}
+
+/*
+ * Synthetic filterbank block, layout [c][ld], samples t0..t0+n-1 of a stream.
+ * raw = gain_c * (z + sum of pulse amplitudes covering (c,t)) + offset_c.
+ * Pulses: arrays of dm, start sample, width, amplitude (normalised units);
+ * a pulse covers [start + delay_c(dm), start + delay_c(dm) + width).
+ */
+void oracle_generate(uint64_t seed, int nchans, double fch1, double foff, double tsamp,
+                     int64_t t0, int64_t n, const float *gain, const float *offset, int npulses,
+                     const double *p_dm, const int64_t *p_start, const int32_t *p_width,
+                     const float *p_amp, float *out, int64_t ld) {
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nchans; ++c) {
+    float *row = out + (int64_t)c * ld;
+    float z[4];
+    int64_t q_cached = -1;
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t t = t0 + i;
+      int64_t q = t >> 2;
+      if (q != q_cached) { gauss4(seed, (uint32_t)c, (uint64_t)q, z); q_cached = q; }
+      float v = z[t & 3];
+      for (int p = 0; p < npulses; ++p) {
+        int64_t s = p_start[p] + ns_delay(fch1, foff, c, tsamp, p_dm[p]);
+        if (t >= s && t < s + p_width[p]) v = v + p_amp[p];
+      }
+      row[i] = gain[c] * v + offset[c];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Delay tables                                                        */
+/* ------------------------------------------------------------------ */
+/* North-star table: D[c][d] = round(4148.808*DM_d*(f_c^-2 - f_top^-2)/tsamp), FP64,
+ * f_c = fch1 + foff*c (channel 0 = top), DM_d = lowdm + d*dmstep. */
+void oracle_delay_table(double fch1, double foff, int nchans, double tsamp, double lowdm,
+                        double dmstep, int ndms, int32_t *table) {
+  for (int c = 0; c < nchans; ++c) {
+    double f = fch1 + foff * (double)c;
+    double k = 1.0 / (f * f) - 1.0 / (fch1 * fch1);
+    for (int d = 0; d < ndms; ++d) {
+      double dm = lowdm + dmstep * (double)d;
+      table[(int64_t)c * ndms + d] = (int32_t)round(((KDM_NORTHSTAR * dm) * k) / tsamp);
+    }
+  }
+}
+
+/* Reference dm_shifts (multibeam_dedispersion_manager.cpp:325-326,366-369):
+ * float F1 = fch1 + foff*j (float arithmetic), dmdelay evaluated in double,
+ * stored as float. */
+void oracle_dm_shifts_ref(float fch1, float foff, int nchans, float *shifts) {
+  for (int j = 0; j < nchans; ++j) {
+    float F1 = fch1 + foff * (float)j;
+    float F2 = fch1;
+    shifts[j] = (float)(KDM_REFERENCE * ((1.0 / F1 / F1) - (1.0 / F2 / F2)));
+  }
+}
+
+/* Reference delay table (multibeam_dedispersion_thread.cu:551-553):
+ * (int)(dm_shifts[c] * (lowdm + d*dmstep) / tsamp), float arithmetic, truncation. */
+void oracle_delay_table_ref(float fch1, float foff, int nchans, float tsamp, float lowdm,
+                            float dmstep, int ndms, int32_t *table) {
+  float *s = (float *)malloc(sizeof(float) * (size_t)nchans);
+  oracle_dm_shifts_ref(fch1, foff, nchans, s);
+  for (int c = 0; c < nchans; ++c)
+    for (int d = 0; d < ndms; ++d) {
+      float dm = lowdm + (float)d * dmstep;
+      float v = s[c] * dm;
+      v = v / tsamp;
+      table[(int64_t)c * ndms + d] = (int32_t)v;
+    }
+  free(s);
+}
+
+/* Reference maxshift for a list of beams (manager.cpp:362-380), including the
+ * round-up to a multiple of 256 that is applied once per beam inside the loop. */
+uint32_t oracle_maxshift_ref(const float *fch1, const float *foff, int nbeams, int nchans,
+                             float tsamp, float lowdm, float dmstep, int ndms) {
+  uint32_t greatest = 0;
+  float *s = (float *)malloc(sizeof(float) * (size_t)nchans);
+  for (int b = 0; b < nbeams; ++b) {
+    oracle_dm_shifts_ref(fch1[b], foff[b], nchans, s);
+    float high_dm = lowdm + dmstep * (float)(ndms - 1);
+    uint32_t ms = (uint32_t)(s[nchans - 1] * high_dm / tsamp);
+    greatest = ms > greatest ? ms : greatest;
+    greatest = (greatest / 256) * 256 + 256;
+  }
+  free(s);
+  return greatest;
+}
+
+/* Reference LDS halo size (thread.cu:542-543), round() of float products. */
+uint32_t oracle_shared_shift_ref(float bottom_shift, float lowdm, float dmstep, int ndms,
+                                 float tsamp, int dedisp_dms) {
+  double a = round(bottom_shift * (lowdm + (float)ndms * dmstep) / tsamp);
+  double b = round(bottom_shift * (lowdm + (float)(ndms - dedisp_dms) * dmstep) / tsamp);
+  return (uint32_t)(a - b);
+}
+
+/* Reference nsamp sizing (manager.cpp:329-347). */
+uint32_t oracle_calculate_nsamp_ref(uint32_t nsamp, uint32_t maxshift, uint32_t nchans,
+                                    uint32_t ndms, uint64_t memory_mb) {
+  if (nsamp == 0)
+    nsamp = (uint32_t)(((double)memory_mb * 1000 * 0.99 / sizeof(float) -
+                        (double)maxshift * nchans) / (double)(nchans + ndms));
+  if (maxshift > nsamp) nsamp = (uint32_t)pow(2, ceil(log((double)maxshift) / log(2)));
+  return nsamp;
+}
+
+/* ------------------------------------------------------------------ */
+/* Brute-force dedispersion                                            */
+/* out[d][t] = sum_c in[c][t + D[c][d]]  (multibeam_dedispersion_kernel.cu:44-95) */
+/* ------------------------------------------------------------------ */
+void oracle_dedisperse_f64(const float *in, int64_t ld, int nchans, int nsamp, int ndms,
+                           const int32_t *table, double *out) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int d = 0; d < ndms; ++d) {
+    double *o = out + (int64_t)d * nsamp;
+    for (int t = 0; t < nsamp; ++t) o[t] = 0.0;
+    for (int c = 0; c < nchans; ++c) {
+      const float *r = in + (int64_t)c * ld + table[(int64_t)c * ndms + d];
+      for (int t = 0; t < nsamp; ++t) o[t] += (double)r[t];
+    }
+  }
+}
+
+/* FP32 accumulation in channel order 0..C-1, one rounding per add: the
+ * arithmetic of the reference kernel.  Multithreaded over (DM, time tile) so
+ * it also serves as the CPU baseline. */
+void oracle_dedisperse_f32seq(const float *in, int64_t ld, int nchans, int nsamp, int ndms,
+                              const int32_t *table, float *out) {
+  const int TT = 2048;
+  int ntiles = (nsamp + TT - 1) / TT;
+  int64_t work = (int64_t)ndms * ntiles;
+#pragma omp parallel
+  {
+    float acc[2048] __attribute__((aligned(64)));
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t w = 0; w < work; ++w) {
+      int d = (int)(w / ntiles);
+      int t0 = (int)(w % ntiles) * TT;
+      int n = nsamp - t0 < TT ? nsamp - t0 : TT;
+      for (int t = 0; t < n; ++t) acc[t] = 0.0f;
+      for (int c = 0; c < nchans; ++c) {
+        const float *r = in + (int64_t)c * ld + table[(int64_t)c * ndms + d] + t0;
+#pragma omp simd
+        for (int t = 0; t < n; ++t) acc[t] += r[t];
+      }
+      memcpy(out + (int64_t)d * nsamp + t0, acc, sizeof(float) * (size_t)n);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* IEEE binary16 conversion (round to nearest, ties to even)           */
+/* ------------------------------------------------------------------ */
+uint16_t oracle_f32_to_f16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return (uint16_t)(sign | (ax > 0x7F800000u ? 0x7E00u : 0x7C00u));
+  if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u); /* rounds to >= 65520: inf */
+  if (ax < 0x38800000u) {                                   /* subnormal half or zero */
+    if (ax < 0x33000000u) return (uint16_t)sign;            /* < 2^-25: rounds to 0 */
+    uint32_t e = ax >> 23;
+    uint32_t m = (ax & 0x7FFFFFu) | 0x800000u;
+    /* value = m * 2^(e-150); in half subnormal units of 2^-24 that is m >> (126-e) */
+    uint32_t shift = 126u - e;
+    uint32_t rem = m & ((1u << shift) - 1u);
+    uint32_t half = 1u << (shift - 1u);
+    uint32_t q = m >> shift;
+    if (rem > half || (rem == half && (q & 1u))) q += 1u;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t e = (ax >> 23) - 112u; /* rebias 127 -> 15 */
+  uint32_t m = ax & 0x7FFFFFu;
+  uint32_t h = (e << 10) | (m >> 13);
+  uint32_t rem = m & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1u;
+  return (uint16_t)(sign | h);
+}
+
+float oracle_f16_to_f32(uint16_t h) {
+  uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+  uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu, x;
+  if (e == 0) {
+    if (m == 0) x = sign;
+    else {
+      float v = (float)m * (1.0f / 16777216.0f);
+      memcpy(&x, &v, 4);
+      x |= sign;
+    }
+  } else if (e == 31) x = sign | 0x7F800000u | (m << 13);
+  else x = sign | ((e + 112u) << 23) | (m << 13);
+  float f;
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+void oracle_f32_to_f16_array(const float *in, uint16_t *out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = oracle_f32_to_f16(in[i]);
+}
+
+void oracle_f16_to_f32_array(const uint16_t *in, float *out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = oracle_f16_to_f32(in[i]);
+}
+
+/* FP16-stored input, FP32 accumulation in channel order. */
+void oracle_dedisperse_f16(const uint16_t *in, int64_t ld, int nchans, int nsamp, int ndms,
+                           const int32_t *table, float *out) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int d = 0; d < ndms; ++d) {
+    float *o = out + (int64_t)d * nsamp;
+    for (int t = 0; t < nsamp; ++t) o[t] = 0.0f;
+    for (int c = 0; c < nchans; ++c) {
+      const uint16_t *r = in + (int64_t)c * ld + table[(int64_t)c * ndms + d];
+      for (int t = 0; t < nsamp; ++t) o[t] += oracle_f16_to_f32(r[t]);
+    }
+  }
+}
+
+/* uint8 input: value = u * (8/255) - 4 in normalised units.  The sum over
+ * channels is an exact integer S; out = (float)(S * (8/255) - 4*C). */
+#define U8_SCALE (8.0 / 255.0)
+#define U8_OFFSET (-4.0)
+void oracle_dedisperse_u8(const uint8_t *in, int64_t ld, int nchans, int nsamp, int ndms,
+                          const int32_t *table, float *out) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int d = 0; d < ndms; ++d) {
+    uint32_t *s = (uint32_t *)calloc((size_t)nsamp, sizeof(uint32_t));
+    for (int c = 0; c < nchans; ++c) {
+      const uint8_t *r = in + (int64_t)c * ld + table[(int64_t)c * ndms + d];
+      for (int t = 0; t < nsamp; ++t) s[t] += r[t];
+    }
+    for (int t = 0; t < nsamp; ++t)
+      out[(int64_t)d * nsamp + t] = (float)((double)s[t] * U8_SCALE + (double)nchans * U8_OFFSET);
+    free(s);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Reductions with a fixed order (the order the GPU kernels use):       */
+/* lane i of 256 sums elements i, i+256, ... sequentially; then a binary */
+/* tree p[i] += p[i+s] for s = 128, 64, ..., 1.                          */
+/* ------------------------------------------------------------------ */
+static void reduce_r256_f(const float *x, int64_t n, double *s1, double *s2) {
+  double p1[R256], p2[R256];
+  for (int i = 0; i < R256; ++i) {
+    double a = 0.0, b = 0.0;
+    for (int64_t j = i; j < n; j += R256) {
+      double v = (double)x[j];
+      a = a + v;
+      b = b + v * v;
+    }
+    p1[i] = a;
+    p2[i] = b;
+  }
+  for (int s = R256 / 2; s > 0; s >>= 1)
+    for (int i = 0; i < s; ++i) {
+      p1[i] = p1[i] + p1[i + s];
+      p2[i] = p2[i] + p2[i + s];
+    }
+  *s1 = p1[0];
+  *s2 = p2[0];
+}
+
+static void reduce_r256_d(const double *x, int64_t n, double *s1, double *s2) {
+  double p1[R256], p2[R256];
+  for (int i = 0; i < R256; ++i) {
+    double a = 0.0, b = 0.0;
+    for (int64_t j = i; j < n; j += R256) {
+      a = a + x[j];
+      b = b + x[j] * x[j];
+    }
+    p1[i] = a;
+    p2[i] = b;
+  }
+  for (int s = R256 / 2; s > 0; s >>= 1)
+    for (int i = 0; i < s; ++i) {
+      p1[i] = p1[i] + p1[i + s];
+      p2[i] = p2[i] + p2[i + s];
+    }
+  *s1 = p1[0];
+  *s2 = p2[0];
+}
+
+static inline void moments(double s1, double s2, int64_t n, double *mean, double *stdv) {
+  double m = s1 / (double)n;
+  double v = s2 / (double)n - m * m;
+  *mean = m;
+  *stdv = v > 0.0 ? sqrt(v) : 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Bandpass normalisation: per-channel FP64 mean/std over n samples,     */
+/* then v = (x - mean) * (1/std) in FP64, stored as f32, f16 or u8.      */
+/* ------------------------------------------------------------------ */
+void oracle_channel_stats(const float *raw, int64_t ld, int nchans, int64_t n, double *mean,
+                          double *stdv) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int c = 0; c < nchans; ++c) {
+    double s1, s2;
+    reduce_r256_f(raw + (int64_t)c * ld, n, &s1, &s2);
+    moments(s1, s2, n, &mean[c], &stdv[c]);
+  }
+}
+
+static inline double norm_value(float x, double mean, double inv) {
+  return ((double)x - mean) * inv;
+}
+
+static inline double inv_std(double s) { return s > 0.0 ? 1.0 / s : 0.0; }
+
+void oracle_normalise_f32(const float *raw, int64_t ld, int nchans, int64_t n, const double *mean,
+                          const double *stdv, float *out, int64_t ldo) {
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nchans; ++c) {
+    double inv = inv_std(stdv[c]);
+    for (int64_t t = 0; t < n; ++t)
+      out[(int64_t)c * ldo + t] = (float)norm_value(raw[(int64_t)c * ld + t], mean[c], inv);
+  }
+}
+
+void oracle_normalise_f16(const float *raw, int64_t ld, int nchans, int64_t n, const double *mean,
+                          const double *stdv, uint16_t *out, int64_t ldo) {
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nchans; ++c) {
+    double inv = inv_std(stdv[c]);
+    for (int64_t t = 0; t < n; ++t)
+      out[(int64_t)c * ldo + t] =
+          oracle_f32_to_f16((float)norm_value(raw[(int64_t)c * ld + t], mean[c], inv));
+  }
+}
+
+static inline uint8_t quant_u8(double v) {
+  if (v < -4.0) v = -4.0;
+  if (v > 4.0) v = 4.0;
+  double q = floor((v + 4.0) * 31.875 + 0.5);
+  return (uint8_t)q;
+}
+
+void oracle_normalise_u8(const float *raw, int64_t ld, int nchans, int64_t n, const double *mean,
+                         const double *stdv, uint8_t *out, int64_t ldo) {
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nchans; ++c) {
+    double inv = inv_std(stdv[c]);
+    for (int64_t t = 0; t < n; ++t)
+      out[(int64_t)c * ldo + t] = quant_u8(norm_value(raw[(int64_t)c * ld + t], mean[c], inv));
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Detection (north star): per-block FP64 mean/RMS, dyadic boxcars,      */
+/* best width per sample, threshold.                                     */
+/* ------------------------------------------------------------------ */
+/* Block (d, b) covers samples [b*L, min((b+1)*L, nsamp)) of row d. */
+void oracle_block_stats(const double *x, int nrows, int nsamp, int L, double *mean, double *stdv) {
+  int nb = (nsamp + L - 1) / L;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int d = 0; d < nrows; ++d)
+    for (int b = 0; b < nb; ++b) {
+      int64_t t0 = (int64_t)b * L;
+      int64_t n = nsamp - t0 < L ? nsamp - t0 : L;
+      double s1, s2;
+      reduce_r256_d(x + (int64_t)d * nsamp + t0, n, &s1, &s2);
+      moments(s1, s2, n, &mean[(int64_t)d * nb + b], &stdv[(int64_t)d * nb + b]);
+    }
+}
+
+/* sqrt(2^k) for k = 0..6, correctly rounded. */
+static const double SQRT_W[7] = {1.0, 1.4142135623730951, 2.0, 2.8284271247461903,
+                                 4.0, 5.656854249492381, 8.0};
+
+/*
+ * For each row d and sample t: S_0(t) = x(t), S_{k+1}(t) = S_k(t) + S_k(t + 2^k)
+ * (FP64, this exact pairing), snr_k(t) = (S_k(t) - 2^k*mean)/(std*sqrt(2^k)) with
+ * the stats of the block holding t, for every k with t + 2^k <= nsamp and
+ * k < nwidths.  Best width = first k reaching the maximum.  A sample is a
+ * candidate when the best snr >= thresh.  Candidates are written in (d, t)
+ * order; returns the total count (entries beyond cap are not written).
+ */
+int64_t oracle_detect(const double *x, int nrows, int nsamp, int L, const double *mean,
+                      const double *stdv, int nwidths, double thresh, int64_t cap, int32_t *cd,
+                      int32_t *ct, int32_t *ck, float *csnr) {
+  int nb = (nsamp + L - 1) / L;
+  int64_t total = 0;
+  double *lev = (double *)malloc(sizeof(double) * (size_t)nsamp);
+  double *best = (double *)malloc(sizeof(double) * (size_t)nsamp);
+  int *bestk = (int *)malloc(sizeof(int) * (size_t)nsamp);
+  for (int d = 0; d < nrows; ++d) {
+    const double *r = x + (int64_t)d * nsamp;
+    for (int t = 0; t < nsamp; ++t) { lev[t] = r[t]; best[t] = -INFINITY; bestk[t] = -1; }
+    for (int k = 0; k < nwidths; ++k) {
+      int w = 1 << k;
+      if (k > 0) {
+        int h = w >> 1;
+        for (int t = 0; t + w <= nsamp; ++t) lev[t] = lev[t] + lev[t + h];
+      }
+      for (int t = 0; t + w <= nsamp; ++t) {
+        int b = t / L;
+        double m = mean[(int64_t)d * nb + b], s = stdv[(int64_t)d * nb + b];
+        if (!(s > 0.0)) continue;
+        double snr = (lev[t] - (double)w * m) / (s * SQRT_W[k]);
+        if (snr > best[t]) { best[t] = snr; bestk[t] = k; }
+      }
+    }
+    for (int t = 0; t < nsamp; ++t)
+      if (bestk[t] >= 0 && best[t] >= thresh) {
+        if (total < cap) {
+          cd[total] = d; ct[total] = t; ck[total] = bestk[t]; csnr[total] = (float)best[t];
+        }
+        total++;
+      }
+  }
+  free(lev); free(best); free(bestk);
+  return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* Reference global statistics + threshold (process_brute_clustering,    */
+/* multibeam_dedispersion_output.cpp:206-248): mean and rms over all      */
+/* M*N values in double, candidate if x - mean >= rms*thresh.             */
+/* ------------------------------------------------------------------ */
+void oracle_global_stats(const float *x, int64_t n, double *mean, double *rms) {
+  double m = 0.0, s = 0.0;
+  for (int64_t j = 0; j < n; ++j) {
+    m += x[j];
+    s += pow(x[j], 2);
+  }
+  m /= (double)n;
+  *mean = m;
+  *rms = sqrt(s / (double)n - m * m);
+}
+
+int64_t oracle_threshold_ref(const float *x, int ndms, int nsamp, double mean, double rms,
+                             float thresh, int64_t cap, int32_t *cd, int32_t *ct, float *csnr) {
+  int64_t total = 0;
+  for (int k = 0; k < ndms; ++k)
+    for (int l = 0; l < nsamp; ++l) {
+      float v = x[(int64_t)k * nsamp + l];
+      if (v - mean >= rms * thresh) {
+        if (total < cap) { cd[total] = k; ct[total] = l; csnr[total] = (float)((v - mean) / rms); }
+        total++;
+      }
+    }
+  return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* DBSCAN, 2-D north-star variant: points (d, t), Chebyshev distance,    */
+/* neighbour iff |dd| <= eps and |dt| <= eps, core iff the neighbourhood  */
+/* (self included) has >= minpts points.  Clusters numbered 0.. in order  */
+/* of their first core point; a border point joins the lowest-numbered    */
+/* cluster with a core point in reach; noise = -1.                        */
+/* ------------------------------------------------------------------ */
+int oracle_dbscan2d(const int32_t *d, const int32_t *t, int n, int eps, int minpts, int32_t *label) {
+  char *core = (char *)calloc((size_t)n, 1);
+  int *queue = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    int cnt = 0;
+    for (int j = 0; j < n; ++j)
+      if (abs(d[i] - d[j]) <= eps && abs(t[i] - t[j]) <= eps) cnt++;
+    core[i] = cnt >= minpts;
+    label[i] = -1;
+  }
+  int nclusters = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!core[i] || label[i] != -1) continue;
+    int id = nclusters++;
+    int qh = 0, qt = 0;
+    label[i] = id;
+    queue[qt++] = i;
+    while (qh < qt) {
+      int p = queue[qh++];
+      for (int j = 0; j < n; ++j) {
+        if (abs(d[p] - d[j]) > eps || abs(t[p] - t[j]) > eps) continue;
+        if (label[j] != -1) continue;
+        label[j] = id;
+        if (core[j]) queue[qt++] = j;
+      }
+    }
+  }
+  free(core);
+  free(queue);
+  return nclusters;
+}
+
+/* ------------------------------------------------------------------ */
+/* DBSCAN, reference 3-D box variant (dbscan.cpp:194-276, 390-415).      */
+/* cluster: in/out, 0 = unassigned on entry; ids from 1; noise -1.        */
+/* ------------------------------------------------------------------ */
+static unsigned ref_neighbours(const double *tm, const double *dm, const double *snr,
+                               const int32_t *cluster, unsigned n, unsigned index, char *nb,
+                               const char *visited, float min_time, float min_dm, float min_snr) {
+  unsigned count = 0;
+  for (unsigned i = 0; i < n; ++i) {
+    if (index != i && cluster[i] <= 0) {
+      float x = (float)fabs(tm[index] - tm[i]);
+      float y = (float)fabs(dm[index] - dm[i]);
+      float z = (float)fabs(snr[index] - snr[i]);
+      if (x < min_time && y < min_dm && z < min_snr && nb[i] == 0 && visited[i] == 0) {
+        nb[i] = 1;
+        count++;
+      }
+    }
+  }
+  return count;
+}
+
+int oracle_dbscan_ref(const double *tm, const double *dm, const double *snr, unsigned n,
+                      float min_time, float min_dm, float min_snr, unsigned min_points,
+                      int32_t *cluster) {
+  char *nb = (char *)calloc(n ? n : 1, 1);
+  char *visited = (char *)calloc(n ? n : 1, 1);
+  int nclusters = 0;
+  for (unsigned i = 0; i < n; ++i) cluster[i] = 0;
+  for (unsigned i = 0; i < n; ++i) {
+    if (visited[i]) continue;
+    visited[i] = 1;
+    unsigned found = ref_neighbours(tm, dm, snr, cluster, n, i, nb, visited, min_time, min_dm, min_snr);
+    if (found < min_points) {
+      cluster[i] = -1;
+      memset(nb, 0, n);
+      continue;
+    }
+    int id = ++nclusters;
+    cluster[i] = id;
+    while (found != 0) {
+      unsigned j = 0;
+      while (nb[j] == 0) {
+        j++;
+        if (j >= n) j = 0;
+      }
+      if (!visited[j]) {
+        visited[j] = 1;
+        found += ref_neighbours(tm, dm, snr, cluster, n, j, nb, visited, min_time, min_dm, min_snr);
+      }
+      if (cluster[j] == 0) cluster[j] = id;
+      nb[j] = 0;
+      found--;
+    }
+  }
+  free(nb);
+  free(visited);
+  return nclusters;
+}
+
+/* ------------------------------------------------------------------ */
+/* Median of five, restating the selection network of                    */
+/* src/mdsm/dedispersion_output.cpp:14-53 operation for operation.        */
+/* ------------------------------------------------------------------ */
+float oracle_median5(float n1, float n2, float n3, float n4, float n5) {
+  float *a = &n1, *b = &n2, *c = &n3, *d = &n4, *e = &n5, *tmp;
+  if (*b < *a) { tmp = a; a = b; b = tmp; }
+  if (*d < *c) { tmp = c; c = d; d = tmp; }
+  if (*c < *a) { tmp = b; b = d; d = tmp; c = a; }
+  a = e;
+  if (*b < *a) { tmp = a; a = b; b = tmp; }
+  if (*a < *c) { tmp = b; b = d; d = tmp; a = c; }
+  if (*d < *a) return *d;
+  return *a;
+}
+
+/* 5-tap running median per DM row, out-of-place, edge samples replicated
+ * (the intent of median_filter, multibeam_dedispersion_kernel.cu:98-157). */
+void oracle_median_filter5(const float *x, int nrows, int nsamp, float *out) {
+  for (int d = 0; d < nrows; ++d) {
+    const float *r = x + (int64_t)d * nsamp;
+    for (int t = 0; t < nsamp; ++t) {
+      float v[5];
+      for (int k = -2; k <= 2; ++k) {
+        int i = t + k;
+        i = i < 0 ? 0 : (i >= nsamp ? nsamp - 1 : i);
+        v[k + 2] = r[i];
+      }
+      /* insertion sort, take the middle */
+      for (int i = 1; i < 5; ++i)
+        for (int j = i; j > 0 && v[j] < v[j - 1]; --j) { float s = v[j]; v[j] = v[j - 1]; v[j - 1] = s; }
+      out[(int64_t)d * nsamp + t] = v[2];
+    }
+  }
+}
+
+/* Legacy detector (dedispersion_output.cpp:135-213, mean == NULL branch):
+ * mean/rms from 32K-value blocks, median of five, threshold with thisdm > 1. */
+int64_t oracle_process_brute_legacy(const float *x, int ndms, int nsamp, float lowdm, float dmstep,
+                                    float thresh, int64_t cap, int32_t *cd, int32_t *ct, float *csnr,
+                                    float *out_mean, float *out_rms) {
+  unsigned vals = (unsigned)nsamp * (unsigned)ndms;
+  unsigned mod_factor = vals < 32 * 1024 ? vals : 32 * 1024;
+  unsigned iters = 0, j;
+  double mean2 = 0, total, total2;
+  float localmean = 0, localrms;
+  while (1) {
+    total = 0;
+    total2 = 0;
+    for (j = 0; j < mod_factor; j++) {
+      total += x[iters * mod_factor + j];
+      total2 += pow(x[iters * mod_factor + j], 2);
+    }
+    localmean = (float)((double)localmean + total / j);
+    mean2 += (total2 / j);
+    iters++;
+    if (iters * mod_factor + j >= vals) break;
+  }
+  localmean /= (float)iters;
+  float lm2 = localmean * localmean;
+  localrms = (float)sqrt(mean2 / iters - (double)lm2);
+  *out_mean = localmean;
+  *out_rms = localrms;
+  int64_t total_c = 0;
+  for (int k = 0; k < ndms; ++k) {
+    const float *r = x + (int64_t)k * nsamp;
+    for (int l = 0; l < nsamp - 4; ++l) {
+      float med = oracle_median5(r[l], r[l + 1], r[l + 2], r[l + 3], r[l + 4]);
+      float thisdm = lowdm + k * dmstep;
+      if (med - localmean >= localrms * thresh && thisdm > 1.0) {
+        if (total_c < cap) { cd[total_c] = k; ct[total_c] = l; csnr[total_c] = med / localrms; }
+        total_c++;
+      }
+    }
+  }
+  return total_c;
+}
+
+/* Linear detrend and normalisation of consecutive segments of length L
+ * (detrend_normalise, multibeam_dedispersion_kernel.cu:207-295), FP64
+ * accumulation: x_i = -L/2 + 0.5 + i; fit y = a + b x by least squares
+ * (a = mean(y), b = sum(xy)/sum(xx)); subtract; divide by sqrt(mean(r^2)). */
+void oracle_detrend_normalise(const float *x, int64_t nseg, int L, float *out) {
+  for (int64_t s = 0; s < nseg; ++s) {
+    const float *y = x + s * L;
+    double sy = 0, sxy = 0, sxx = 0;
+    for (int i = 0; i < L; ++i) {
+      double xi = -L / 2.0 + 0.5 + i;
+      sy += y[i];
+      sxy += xi * y[i];
+      sxx += xi * xi;
+    }
+    double a = sy / L, b = sxy / sxx, ss = 0;
+    for (int i = 0; i < L; ++i) {
+      double xi = -L / 2.0 + 0.5 + i;
+      double r = (double)y[i] - (a + b * xi);
+      ss += r * r;
+    }
+    double sd = sqrt(ss / L);
+    for (int i = 0; i < L; ++i) {
+      double xi = -L / 2.0 + 0.5 + i;
+      out[s * L + i] = (float)(((double)y[i] - (a + b * xi)) / sd);
+    }
+  }
+}
+
+int oracle_num_threads(void) { return omp_get_max_threads(); }
