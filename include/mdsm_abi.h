@@ -1,0 +1,238 @@
+/*
+ * mdsm_abi.h -- C ABI of the MI355X (gfx950) brute-force dedispersion and
+ * single-pulse detection library (libmdsm_amd.so).
+ *
+ * Plain C: pointers, sizes and status codes only.  No entry point calls
+ * exit(); every one returns an MDSM_* status and leaves a message for
+ * mdsm_last_error().  Device pointers are HIP device allocations; "stream"
+ * arguments are hipStream_t passed as void* (NULL = the null stream).
+ *
+ * Reference interfaces replaced (paths relative to the MDSM source tree):
+ *   src/mdsm-multibeam/multibeam_dedispersion_wrapper.cu:4-23   the C seam
+ *       (call_dedisperse, call_initialise_devices, call_allocate*Buffer)
+ *   src/mdsm-multibeam/multibeam_dedispersion_thread.cu:204-221 shared_brute_force
+ *   src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:44-95   shared_dedispersion
+ *   src/mdsm-multibeam/multibeam_dedispersion_output.cpp:188-321 process_brute_clustering
+ *   src/mdsm-multibeam/dbscan.cpp:194-276                        DBScan::performClustering
+ * Individual citations are given per declaration below.
+ */
+#ifndef MDSM_ABI_H
+#define MDSM_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+  MDSM_OK = 0,
+  MDSM_ERR_ARG = 1,      /* invalid argument or survey parameters */
+  MDSM_ERR_HIP = 2,      /* a HIP runtime call failed */
+  MDSM_ERR_NOMEM = 3,    /* device or host allocation failed */
+  MDSM_ERR_OVERFLOW = 4, /* a bounded output (candidate list) overflowed */
+  MDSM_ERR_STATE = 5     /* call out of sequence */
+};
+
+/* Input sample storage types. */
+enum { MDSM_F32 = 0, MDSM_F16 = 1, MDSM_U8 = 2 };
+
+/* Delay-table conventions.
+ * NORTHSTAR: D[c][d] = round(4148.808*DM_d*(f_c^-2 - f_top^-2)/tsamp), FP64.
+ * REFERENCE: (int)(dm_shifts[c]*(lowdm + d*dmstep)/tsamp) in float with
+ *   dm_shifts[c] = (float)(4148.741601*(F1^-2 - F2^-2)), F1 = fch1 + foff*c in
+ *   float (multibeam_dedispersion_manager.cpp:325-326,366-369;
+ *   multibeam_dedispersion_thread.cu:551-553). */
+enum { MDSM_DELAY_NORTHSTAR = 0, MDSM_DELAY_REFERENCE = 1 };
+
+/* Survey parameters of one beam: the <observation> XML fields of the
+ * reference (SURVEY/BEAM, src/mdsm-multibeam/survey.h:8-88). */
+typedef struct {
+  double fch1;      /* MHz, centre of channel 0 (the top of the band) */
+  double foff;      /* MHz per channel, negative (descending) */
+  double tsamp;     /* seconds */
+  double lowdm;     /* pc cm^-3 */
+  double dmstep;    /* pc cm^-3 */
+  uint32_t nchans;
+  uint32_t ndms;    /* "tdms" / numDMs */
+  uint32_t nsamp;   /* output samples per chunk */
+  int32_t delay_mode;
+} mdsm_survey_t;
+
+/* One detection: DM trial index, first sample of the boxcar, log2 of the
+ * boxcar width, signal-to-noise ratio. */
+typedef struct {
+  int32_t dm_idx;
+  int32_t sample;
+  int32_t width_log2;
+  float snr;
+} mdsm_cand_t;
+
+const char *mdsm_last_error(void);
+int mdsm_version(void);
+
+/* --------------------------------------------------------- device helpers
+ * Replace call_initialise_devices / call_allocateInputBuffer /
+ * call_allocateOutputBuffer (multibeam_dedispersion_wrapper.cu:13-23). */
+int mdsm_device_count(int *count);
+int mdsm_set_device(int device);
+int mdsm_malloc(void **ptr, size_t bytes);
+int mdsm_free(void *ptr);
+int mdsm_host_malloc(void **ptr, size_t bytes); /* pinned host memory */
+int mdsm_host_free(void *ptr);
+int mdsm_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int mdsm_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int mdsm_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+int mdsm_memset(void *ptr, int value, size_t bytes, void *stream);
+int mdsm_stream_create(void **stream);
+int mdsm_stream_destroy(void *stream);
+int mdsm_stream_synchronize(void *stream);
+int mdsm_device_synchronize(void);
+/* Event timing on a stream: elapsed milliseconds between two records. */
+int mdsm_event_create(void **event);
+int mdsm_event_destroy(void *event);
+int mdsm_event_record(void *event, void *stream);
+int mdsm_event_elapsed(void *start, void *stop, float *ms);
+
+/* ------------------------------------------------------------ delay table
+ * dmdelay + per-beam dm_shifts + all_shifts + maxshift
+ * (multibeam_dedispersion_manager.cpp:325-326,366-387;
+ *  multibeam_dedispersion_thread.cu:546-559).  Host only.
+ * table: int32[nchans][ndms] (may be NULL); maxshift: largest delay (may be NULL). */
+int mdsm_delay_table(const mdsm_survey_t *s, int32_t *table, uint32_t *maxshift);
+
+/* ------------------------------------------------------------------- plan
+ * A plan owns one beam's device delay table and launch configuration.
+ * Input rows are [nchans][ld] with ld >= nsamp + maxshift (the overlap-save
+ * layout of multibeam_dedispersion_thread.cu:636-673); output rows are
+ * [ndms][nsamp] float32. */
+typedef struct mdsm_plan_s *mdsm_plan;
+int mdsm_plan_create(const mdsm_survey_t *s, int in_dtype, mdsm_plan *plan);
+int mdsm_plan_destroy(mdsm_plan plan);
+/* maxshift of the plan's table and the minimum input row length (nsamp+maxshift). */
+int mdsm_plan_info(mdsm_plan plan, uint32_t *maxshift, uint64_t *min_ld);
+/* Device pointer to the plan's int32[nchans][ndms] delay table. */
+int mdsm_plan_device_table(mdsm_plan plan, const int32_t **table);
+/* Kernel tile variant (0 = autotuned default); see mdsm_tile_variants(). */
+int mdsm_plan_set_variant(mdsm_plan plan, int variant);
+int mdsm_plan_get_variant(mdsm_plan plan, int *variant);
+int mdsm_tile_variants(int in_dtype, int *count);
+/* Time each tile variant on the given buffers and keep the fastest. */
+int mdsm_plan_autotune(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out, void *stream,
+                       int *best_variant, float *best_ms);
+
+/* ---------------------------------------------------------- dedispersion
+ * out[d][t] = sum_{c=0}^{nchans-1} in[c][t + D[c][d]], accumulated in FP32 in
+ * channel order.  Replaces shared_brute_force / shared_dedispersion
+ * (multibeam_dedispersion_thread.cu:204-221, multibeam_dedispersion_kernel.cu:44-95),
+ * cached_brute_force / cache_dedispersion (:174-201, :8-41) and the legacy
+ * brute_force_dedispersion (src/mdsm/dedispersion_thread.cu:220-243).
+ * d_in dtype is the plan's: F32 floats, F16 halves, or U8 codes whose value
+ * is u*(8/255) - 4 in normalised units (then the sum is exact in integers and
+ * out = (float)(S*(8/255) - 4*nchans)). */
+int mdsm_dedisperse(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out, void *stream);
+
+/* --------------------------------------------------- bandpass normalisation
+ * Per-channel FP64 mean/std over n samples of d_raw[nchans][ld] (fixed
+ * reduction order), then v = (x - mean) * (1/std) stored as F32, F16 or U8
+ * (U8: clip to [-4,4], u = floor((v+4)*255/8 + 0.5)).  The per-channel
+ * statistics of the reference bandpass stage (bandpass_power_sum,
+ * multibeam_dedispersion_kernel.cu:317-350) feed a normaliser here. */
+int mdsm_channel_stats(const float *d_raw, uint64_t ld, uint32_t nchans, uint64_t n,
+                       double *d_mean, double *d_std, void *stream);
+int mdsm_normalise(const float *d_raw, uint64_t ld, uint32_t nchans, uint64_t n,
+                   const double *d_mean, const double *d_std, void *d_out, int out_dtype,
+                   uint64_t ldo, void *stream);
+
+/* ------------------------------------------------------------- detection
+ * Per-block FP64 mean/std of d_x[nrows][nsamp] over blocks of L samples
+ * (block b = samples [bL, min((b+1)L, nsamp))).  Output [nrows][ceil(nsamp/L)]. */
+int mdsm_block_stats(const float *d_x, uint32_t nrows, uint32_t nsamp, uint32_t L,
+                     double *d_mean, double *d_std, void *stream);
+/* Dyadic boxcars of widths 1..2^(nwidths-1) (nwidths <= 7) in FP64, best
+ * width per sample, threshold snr >= thresh; compacted to d_cands (unordered)
+ * with the total count in *d_count (which the call zeroes first on the
+ * stream).  Entries beyond cap are dropped but counted.  Replaces the
+ * threshold scan of process_brute / process_brute_clustering
+ * (multibeam_dedispersion_output.cpp:148-185,234-248). */
+int mdsm_detect(const float *d_x, uint32_t nrows, uint32_t nsamp, uint32_t L,
+                const double *d_mean, const double *d_std, int nwidths, double thresh,
+                mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count, void *stream);
+
+/* Reference-semantics detection (process_brute_clustering,
+ * multibeam_dedispersion_output.cpp:206-248): global FP64 mean and rms over
+ * all nrows*nsamp values, candidate iff x - mean >= rms*thresh, snr =
+ * (x - mean)/rms.  h_mean and h_rms receive the statistics (host). */
+int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, float thresh,
+                       mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count, double *h_mean,
+                       double *h_rms, void *stream);
+
+/* 5-tap running median per row, edges replicated (median_filter,
+ * multibeam_dedispersion_kernel.cu:98-157), out of place. */
+int mdsm_median_filter5(const float *d_x, uint32_t nrows, uint32_t nsamp, float *d_out,
+                        void *stream);
+/* Linear detrend and RMS normalisation of consecutive segments of length L,
+ * in place, FP64 accumulation (detrend_normalise,
+ * multibeam_dedispersion_kernel.cu:207-295). */
+int mdsm_detrend_normalise(float *d_x, uint64_t nseg, uint32_t L, void *stream);
+
+/* -------------------------------------------------------------- clustering
+ * Host side.  mdsm_sort_candidates orders by (dm_idx, sample).
+ * mdsm_dbscan2d: points (dm_idx, sample), Chebyshev distance, neighbour iff
+ * |dd| <= eps and |dt| <= eps; core iff >= minpts points in the neighbourhood
+ * (self included); clusters 0.. in order of their first core point in the
+ * array order; border points join the lowest-numbered reachable cluster;
+ * noise = -1.  mdsm_cluster_peaks: per cluster the highest-snr member (first
+ * in array order on ties) and the member count. */
+int mdsm_sort_candidates(mdsm_cand_t *cands, uint32_t n);
+int mdsm_dbscan2d(const mdsm_cand_t *cands, uint32_t n, int eps, int minpts, int32_t *labels,
+                  uint32_t *nclusters);
+int mdsm_cluster_peaks(const mdsm_cand_t *cands, const int32_t *labels, uint32_t n,
+                       uint32_t nclusters, mdsm_cand_t *peaks, uint32_t *sizes);
+/* Reference 3-D box DBSCAN (DBScan::performClustering + getNeighbours,
+ * dbscan.cpp:194-276,390-415): neighbours strictly within (min_time, min_dm,
+ * min_snr), excluding self and already-assigned or visited points; a point
+ * is a seed iff it has >= min_points such neighbours; ids from 1, noise -1. */
+int mdsm_dbscan_ref(const double *time, const double *dm, const double *snr, uint32_t n,
+                    float min_time, float min_dm, float min_snr, uint32_t min_points,
+                    int32_t *cluster, uint32_t *nclusters);
+
+/* ---------------------------------------------------------- multi-beam batch
+ * Many beams of uneven nchans/nsamp/ndms dedispersed by one persistent
+ * launch: the workgroups pull (beam, tile) items from a device work queue,
+ * largest beams first.  Replaces the per-beam host threads of
+ * multibeam_dedispersion_thread.cu:464-839 (one launch per beam per chunk). */
+typedef struct mdsm_batch_s *mdsm_batch;
+int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, int in_dtype,
+                      mdsm_batch *batch);
+int mdsm_batch_destroy(mdsm_batch batch);
+int mdsm_batch_info(mdsm_batch batch, uint32_t beam, uint32_t *maxshift, uint64_t *min_ld);
+/* d_in[b]: beam b input [nchans_b][ld[b]]; d_out[b]: [ndms_b][nsamp_b].
+ * bind uploads the beam descriptors (synchronous); launch enqueues only a
+ * counter clear and the persistent kernel (no allocation, no host copy), so
+ * it may be captured into a hipGraph and replayed while the bound buffers
+ * stay valid.  dedisperse = bind + launch. */
+int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
+                    float *const *d_out);
+int mdsm_batch_launch(mdsm_batch batch, void *stream);
+int mdsm_batch_dedisperse(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
+                          float *const *d_out, void *stream);
+int mdsm_batch_set_variant(mdsm_batch batch, int variant);
+
+/* ------------------------------------------------------- synthetic input
+ * Seeded synthetic filterbank data generated on the device (Philox4x32-10
+ * counters, Box-Muller normals): raw[c][t0+i] = gain[c]*(z + pulses) +
+ * offset[c] for i < n, written to d_raw[c][i] (row stride ld).  Pulses:
+ * p_start[p] (absolute sample), p_width[p], p_amp[p] (normalised units), and
+ * p_delay[p*nchans + c] (per-channel delay in samples).  Host arrays. */
+int mdsm_generate(uint64_t seed, uint32_t nchans, int64_t t0, uint64_t n, const float *gain,
+                  const float *offset, uint32_t npulses, const int64_t *p_start,
+                  const int32_t *p_width, const float *p_amp, const int64_t *p_delay,
+                  float *d_raw, uint64_t ld, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDSM_ABI_H */

@@ -1,0 +1,440 @@
+"""ctypes binding of libmdsm_amd.so (the C ABI declared in include/mdsm_abi.h).
+
+The HIP kernels and the host logic live in the shared library; this module
+only marshals numpy arrays and device pointers.  Loading fails loudly when
+the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmdsm_amd.so")
+
+MDSM_OK, MDSM_ERR_ARG, MDSM_ERR_HIP, MDSM_ERR_NOMEM, MDSM_ERR_OVERFLOW, MDSM_ERR_STATE = range(6)
+F32, F16, U8 = 0, 1, 2
+DELAY_NORTHSTAR, DELAY_REFERENCE = 0, 1
+_ITEMSIZE = {F32: 4, F16: 2, U8: 1}
+_NP = {F32: np.float32, F16: np.uint16, U8: np.uint8}
+
+CAND_DTYPE = np.dtype([("dm_idx", "<i4"), ("sample", "<i4"), ("width_log2", "<i4"), ("snr", "<f4")])
+
+
+class MdsmError(RuntimeError):
+    pass
+
+
+class Survey(C.Structure):
+    _fields_ = [("fch1", C.c_double), ("foff", C.c_double), ("tsamp", C.c_double),
+                ("lowdm", C.c_double), ("dmstep", C.c_double), ("nchans", C.c_uint32),
+                ("ndms", C.c_uint32), ("nsamp", C.c_uint32), ("delay_mode", C.c_int32)]
+
+    @classmethod
+    def make(cls, fch1, foff, nchans, tsamp, lowdm, dmstep, ndms, nsamp, delay_mode=DELAY_NORTHSTAR):
+        return cls(fch1, foff, tsamp, lowdm, dmstep, nchans, ndms, nsamp, delay_mode)
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise MdsmError(f"{LIB_PATH} is missing: build it with `make -C mdsm_amd/csrc` "
+                        "or __graft_entry__.build()")
+    return C.CDLL(LIB_PATH)
+
+
+_lib = _load()
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_i = C.c_int
+_f = C.c_float
+_d = C.c_double
+_P = C.POINTER
+
+
+def _sig(name, *args):
+    fn = getattr(_lib, name)
+    fn.restype = C.c_int
+    fn.argtypes = list(args)
+    return fn
+
+
+_lib.mdsm_last_error.restype = C.c_char_p
+_lib.mdsm_last_error.argtypes = []
+_sig("mdsm_version")
+_sig("mdsm_device_count", _P(_i))
+_sig("mdsm_set_device", _i)
+_sig("mdsm_malloc", _P(_vp), C.c_size_t)
+_sig("mdsm_free", _vp)
+_sig("mdsm_host_malloc", _P(_vp), C.c_size_t)
+_sig("mdsm_host_free", _vp)
+_sig("mdsm_memcpy_htod", _vp, _vp, C.c_size_t, _vp)
+_sig("mdsm_memcpy_dtoh", _vp, _vp, C.c_size_t, _vp)
+_sig("mdsm_memcpy_dtod", _vp, _vp, C.c_size_t, _vp)
+_sig("mdsm_memset", _vp, _i, C.c_size_t, _vp)
+_sig("mdsm_stream_create", _P(_vp))
+_sig("mdsm_stream_destroy", _vp)
+_sig("mdsm_stream_synchronize", _vp)
+_sig("mdsm_device_synchronize")
+_sig("mdsm_event_create", _P(_vp))
+_sig("mdsm_event_destroy", _vp)
+_sig("mdsm_event_record", _vp, _vp)
+_sig("mdsm_event_elapsed", _vp, _vp, _P(_f))
+_sig("mdsm_delay_table", _P(Survey), _vp, _P(_u32))
+_sig("mdsm_plan_create", _P(Survey), _i, _P(_vp))
+_sig("mdsm_plan_destroy", _vp)
+_sig("mdsm_plan_info", _vp, _P(_u32), _P(_u64))
+_sig("mdsm_plan_device_table", _vp, _P(_vp))
+_sig("mdsm_plan_set_variant", _vp, _i)
+_sig("mdsm_plan_get_variant", _vp, _P(_i))
+_sig("mdsm_tile_variants", _i, _P(_i))
+_sig("mdsm_plan_autotune", _vp, _vp, _u64, _vp, _vp, _P(_i), _P(_f))
+_sig("mdsm_dedisperse", _vp, _vp, _u64, _vp, _vp)
+_sig("mdsm_channel_stats", _vp, _u64, _u32, _u64, _vp, _vp, _vp)
+_sig("mdsm_normalise", _vp, _u64, _u32, _u64, _vp, _vp, _vp, _i, _u64, _vp)
+_sig("mdsm_block_stats", _vp, _u32, _u32, _u32, _vp, _vp, _vp)
+_sig("mdsm_detect", _vp, _u32, _u32, _u32, _vp, _vp, _i, _d, _vp, _u32, _vp, _vp)
+_sig("mdsm_detect_global", _vp, _u32, _u32, _f, _vp, _u32, _vp, _P(_d), _P(_d), _vp)
+_sig("mdsm_median_filter5", _vp, _u32, _u32, _vp, _vp)
+_sig("mdsm_detrend_normalise", _vp, _u64, _u32, _vp)
+_sig("mdsm_sort_candidates", _vp, _u32)
+_sig("mdsm_dbscan2d", _vp, _u32, _i, _i, _vp, _P(_u32))
+_sig("mdsm_cluster_peaks", _vp, _vp, _u32, _u32, _vp, _vp)
+_sig("mdsm_dbscan_ref", _vp, _vp, _vp, _u32, _f, _f, _f, _u32, _vp, _P(_u32))
+_sig("mdsm_batch_create", _P(Survey), _u32, _i, _P(_vp))
+_sig("mdsm_batch_destroy", _vp)
+_sig("mdsm_batch_info", _vp, _u32, _P(_u32), _P(_u64))
+_sig("mdsm_batch_bind", _vp, _P(_vp), _P(_u64), _P(_vp))
+_sig("mdsm_batch_launch", _vp, _vp)
+_sig("mdsm_batch_dedisperse", _vp, _P(_vp), _P(_u64), _P(_vp), _vp)
+_sig("mdsm_batch_set_variant", _vp, _i)
+_sig("mdsm_generate", C.c_uint64, _u32, C.c_int64, _u64, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp,
+     _u64, _vp)
+
+
+def check(rc):
+    if rc != MDSM_OK:
+        msg = _lib.mdsm_last_error().decode(errors="replace")
+        raise MdsmError(f"mdsm status {rc}: {msg}")
+
+
+def lib():
+    return _lib
+
+
+def last_error() -> str:
+    return _lib.mdsm_last_error().decode(errors="replace")
+
+
+def device_count() -> int:
+    n = _i(0)
+    check(_lib.mdsm_device_count(C.byref(n)))
+    return n.value
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+# ------------------------------------------------------------------ memory
+class DeviceBuffer:
+    """An owning HIP device allocation."""
+
+    def __init__(self, nbytes: int):
+        p = _vp()
+        check(_lib.mdsm_malloc(C.byref(p), max(int(nbytes), 1)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, offset: int = 0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        check(_lib.mdsm_memcpy_htod(self.ptr + offset, _ptr(a), a.nbytes, None))
+
+    def download(self, dtype, shape, offset: int = 0):
+        out = np.empty(shape, dtype)
+        assert offset + out.nbytes <= self.nbytes
+        check(_lib.mdsm_memcpy_dtoh(_ptr(out), self.ptr + offset, out.nbytes, None))
+        return out
+
+    def zero(self, stream=None):
+        check(_lib.mdsm_memset(self.ptr, 0, self.nbytes, stream))
+
+    def free(self):
+        if self.ptr:
+            check(_lib.mdsm_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(_lib.mdsm_device_synchronize())
+
+
+class Timer:
+    """HIP-event timer on a stream (milliseconds)."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+        a, b = _vp(), _vp()
+        check(_lib.mdsm_event_create(C.byref(a)))
+        check(_lib.mdsm_event_create(C.byref(b)))
+        self.e0, self.e1 = a.value, b.value
+
+    def start(self):
+        check(_lib.mdsm_event_record(self.e0, self.stream))
+
+    def stop(self) -> float:
+        check(_lib.mdsm_event_record(self.e1, self.stream))
+        ms = _f(0.0)
+        check(_lib.mdsm_event_elapsed(self.e0, self.e1, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            _lib.mdsm_event_destroy(self.e0)
+            _lib.mdsm_event_destroy(self.e1)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------- delay table
+def delay_table(s: Survey):
+    t = np.empty((s.nchans, s.ndms), np.int32)
+    ms = _u32(0)
+    check(_lib.mdsm_delay_table(C.byref(s), _ptr(t), C.byref(ms)))
+    return t, ms.value
+
+
+# -------------------------------------------------------------------- plan
+class Plan:
+    def __init__(self, s: Survey, dtype: int = F32):
+        self.survey = s
+        self.dtype = dtype
+        p = _vp()
+        check(_lib.mdsm_plan_create(C.byref(s), dtype, C.byref(p)))
+        self.handle = p.value
+        ms, ld = _u32(0), _u64(0)
+        check(_lib.mdsm_plan_info(self.handle, C.byref(ms), C.byref(ld)))
+        self.maxshift, self.min_ld = ms.value, ld.value
+
+    def set_variant(self, v: int):
+        check(_lib.mdsm_plan_set_variant(self.handle, v))
+
+    @property
+    def variant(self) -> int:
+        v = _i(0)
+        check(_lib.mdsm_plan_get_variant(self.handle, C.byref(v)))
+        return v.value
+
+    def dedisperse(self, d_in: int, ld: int, d_out: int, stream=None):
+        check(_lib.mdsm_dedisperse(self.handle, d_in, ld, d_out, stream))
+
+    def autotune(self, d_in: int, ld: int, d_out: int, stream=None):
+        v, ms = _i(0), _f(0.0)
+        check(_lib.mdsm_plan_autotune(self.handle, d_in, ld, d_out, stream, C.byref(v), C.byref(ms)))
+        return v.value, ms.value
+
+    def close(self):
+        if self.handle:
+            check(_lib.mdsm_plan_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tile_variants(dtype=F32) -> int:
+    n = _i(0)
+    check(_lib.mdsm_tile_variants(dtype, C.byref(n)))
+    return n.value
+
+
+def dedisperse_host(s: Survey, x: np.ndarray, dtype: int = F32, variant: int = 0):
+    """Upload x [nchans][ld], dedisperse on the GPU, return [ndms][nsamp] float32."""
+    plan = Plan(s, dtype)
+    if variant:
+        plan.set_variant(variant)
+    x = np.ascontiguousarray(x, dtype=_NP[dtype])
+    assert x.shape[0] == s.nchans and x.shape[1] >= plan.min_ld
+    din = DeviceBuffer.from_numpy(x)
+    dout = DeviceBuffer(s.ndms * s.nsamp * 4)
+    plan.dedisperse(din.ptr, x.shape[1], dout.ptr)
+    synchronize()
+    return dout.download(np.float32, (s.ndms, s.nsamp))
+
+
+# ---------------------------------------------------- normalise / detect
+def channel_stats(d_raw: int, ld: int, nchans: int, n: int, d_mean: int, d_std: int, stream=None):
+    check(_lib.mdsm_channel_stats(d_raw, ld, nchans, n, d_mean, d_std, stream))
+
+
+def normalise(d_raw, ld, nchans, n, d_mean, d_std, d_out, out_dtype, ldo, stream=None):
+    check(_lib.mdsm_normalise(d_raw, ld, nchans, n, d_mean, d_std, d_out, out_dtype, ldo, stream))
+
+
+def block_stats(d_x, nrows, nsamp, L, d_mean, d_std, stream=None):
+    check(_lib.mdsm_block_stats(d_x, nrows, nsamp, L, d_mean, d_std, stream))
+
+
+def detect(d_x, nrows, nsamp, L, d_mean, d_std, nwidths, thresh, d_cands, cap, d_count, stream=None):
+    check(_lib.mdsm_detect(d_x, nrows, nsamp, L, d_mean, d_std, nwidths, thresh, d_cands, cap,
+                           d_count, stream))
+
+
+def detect_global(d_x, nrows, nsamp, thresh, d_cands, cap, d_count, stream=None):
+    m, r = _d(0.0), _d(0.0)
+    check(_lib.mdsm_detect_global(d_x, nrows, nsamp, thresh, d_cands, cap, d_count, C.byref(m),
+                                  C.byref(r), stream))
+    return m.value, r.value
+
+
+def median_filter5(d_x, nrows, nsamp, d_out, stream=None):
+    check(_lib.mdsm_median_filter5(d_x, nrows, nsamp, d_out, stream))
+
+
+def detrend_normalise(d_x, nseg, L, stream=None):
+    check(_lib.mdsm_detrend_normalise(d_x, nseg, L, stream))
+
+
+def read_candidates(d_cands: DeviceBuffer, d_count: DeviceBuffer, cap: int):
+    """Download (count, sorted candidate array); raises on overflow."""
+    n = int(d_count.download(np.uint32, (1,))[0])
+    if n > cap:
+        raise MdsmError(f"candidate overflow: {n} > capacity {cap}")
+    c = d_cands.download(CAND_DTYPE, (n,)) if n else np.zeros(0, CAND_DTYPE)
+    sort_candidates(c)
+    return c
+
+
+# ------------------------------------------------------------- clustering
+def sort_candidates(c: np.ndarray):
+    assert c.dtype == CAND_DTYPE and c.flags.c_contiguous
+    check(_lib.mdsm_sort_candidates(_ptr(c), len(c)))
+
+
+def dbscan2d(c: np.ndarray, eps: int = 3, minpts: int = 4):
+    lab = np.empty(len(c), np.int32)
+    n = _u32(0)
+    check(_lib.mdsm_dbscan2d(_ptr(c), len(c), eps, minpts, _ptr(lab), C.byref(n)))
+    return lab, n.value
+
+
+def cluster_peaks(c: np.ndarray, lab: np.ndarray, nclusters: int):
+    peaks = np.zeros(nclusters, CAND_DTYPE)
+    sizes = np.zeros(nclusters, np.uint32)
+    check(_lib.mdsm_cluster_peaks(_ptr(c), _ptr(np.ascontiguousarray(lab, np.int32)), len(c),
+                                  nclusters, _ptr(peaks), _ptr(sizes)))
+    return peaks, sizes
+
+
+def dbscan_ref(time, dm, snr, min_time, min_dm, min_snr, min_points):
+    time = np.ascontiguousarray(time, np.float64)
+    dm = np.ascontiguousarray(dm, np.float64)
+    snr = np.ascontiguousarray(snr, np.float64)
+    lab = np.empty(len(time), np.int32)
+    n = _u32(0)
+    check(_lib.mdsm_dbscan_ref(_ptr(time), _ptr(dm), _ptr(snr), len(time), min_time, min_dm,
+                               min_snr, min_points, _ptr(lab), C.byref(n)))
+    return lab, n.value
+
+
+# ------------------------------------------------------------------ batch
+class Batch:
+    def __init__(self, surveys, dtype=F32):
+        arr = (Survey * len(surveys))(*surveys)
+        p = _vp()
+        check(_lib.mdsm_batch_create(arr, len(surveys), dtype, C.byref(p)))
+        self.handle = p.value
+        self.surveys = list(surveys)
+        self.dtype = dtype
+        self.min_ld = []
+        self.maxshift = []
+        for b in range(len(surveys)):
+            ms, ld = _u32(0), _u64(0)
+            check(_lib.mdsm_batch_info(self.handle, b, C.byref(ms), C.byref(ld)))
+            self.maxshift.append(ms.value)
+            self.min_ld.append(ld.value)
+
+    def set_variant(self, v):
+        check(_lib.mdsm_batch_set_variant(self.handle, v))
+
+    def bind(self, d_in, lds, d_out):
+        n = len(self.surveys)
+        self._keep = ((_vp * n)(*d_in), (_u64 * n)(*lds), (_vp * n)(*d_out))
+        check(_lib.mdsm_batch_bind(self.handle, *self._keep))
+
+    def launch(self, stream=None):
+        check(_lib.mdsm_batch_launch(self.handle, stream))
+
+    def dedisperse(self, d_in, lds, d_out, stream=None):
+        self.bind(d_in, lds, d_out)
+        self.launch(stream)
+
+    def close(self):
+        if self.handle:
+            check(_lib.mdsm_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -------------------------------------------------------------- generator
+@dataclass
+class Pulse:
+    dm: float
+    start: int
+    width: int
+    amp: float
+
+
+def pulse_delays(s: Survey, dms):
+    """Per-channel delays (north-star formula) for arbitrary DMs, via the
+    library's own table builder (a one-trial survey per DM)."""
+    out = np.zeros((len(dms), s.nchans), np.int64)
+    for i, dm in enumerate(dms):
+        one = Survey.make(s.fch1, s.foff, s.nchans, s.tsamp, float(dm), 0.0, 1, 1)
+        t, _ = delay_table(one)
+        out[i] = t[:, 0]
+    return out
+
+
+def generate(seed, s: Survey, t0, n, d_raw, ld, gain, offset, pulses=(), stream=None):
+    pulses = list(pulses)
+    gain = np.ascontiguousarray(gain, np.float32)
+    offset = np.ascontiguousarray(offset, np.float32)
+    if pulses:
+        st = np.array([p.start for p in pulses], np.int64)
+        w = np.array([p.width for p in pulses], np.int32)
+        a = np.array([p.amp for p in pulses], np.float32)
+        dl = np.ascontiguousarray(pulse_delays(s, [p.dm for p in pulses]))
+        args = (_ptr(st), _ptr(w), _ptr(a), _ptr(dl))
+    else:
+        args = (None, None, None, None)
+    check(_lib.mdsm_generate(seed, s.nchans, t0, n, _ptr(gain), _ptr(offset), len(pulses), *args,
+                             d_raw, ld, stream))

@@ -1,0 +1,179 @@
+// batch.cpp -- multi-beam batching: many beams of uneven nchans / nsamp /
+// ndms dedispersed by one persistent launch that drains a device work queue.
+//
+// The reference runs one host pthread per beam, each launching
+// shared_dedispersion on its own slice of device memory
+// (src/mdsm-multibeam/multibeam_dedispersion_thread.cu:464-839,
+// multibeam_dedispersion_manager.cpp:495-512).  Here every beam's tiles are
+// items of one queue: a beam descriptor table in device memory, items
+// ordered largest beam first, one atomic counter.  mdsm_batch_launch issues
+// only a counter-clear kernel and the persistent kernel, so it can be
+// captured in a hipGraph and replayed per chunk.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "internal.h"
+
+using namespace mdsm;
+
+struct mdsm_batch_s {
+  int dtype = MDSM_F32;
+  int variant = 0;
+  std::vector<mdsm_survey_t> beams;
+  std::vector<uint32_t> maxshift;
+  std::vector<std::vector<int32_t>> h_tables;
+  std::vector<int32_t *> d_tables;
+  std::vector<std::vector<uint32_t>> row_words;  // [beam][variant]
+  std::vector<uint32_t> order;                   // processing order of beams
+  BeamDesc *d_desc = nullptr;
+  uint32_t *d_counter = nullptr;
+  bool bound = false;
+  uint32_t bound_variant = 0;
+  uint32_t total_items = 0;
+  uint32_t max_row_words = 0;
+  std::vector<BeamDesc> h_desc;
+};
+
+static int batch_variant(const mdsm_batch_s *b) { return b->variant ? b->variant : 2; }
+
+static void free_batch(mdsm_batch_s *b) {
+  for (auto *p : b->d_tables)
+    if (p) (void)hipFree(p);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_counter) (void)hipFree(b->d_counter);
+  delete b;
+}
+
+extern "C" int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, int in_dtype,
+                                 mdsm_batch *batch) {
+  MDSM_REQUIRE(batch && beams && nbeams > 0, MDSM_ERR_ARG, "bad arguments");
+  MDSM_REQUIRE(in_dtype == MDSM_F32 || in_dtype == MDSM_F16 || in_dtype == MDSM_U8, MDSM_ERR_ARG,
+               "unknown input dtype %d", in_dtype);
+  *batch = nullptr;
+  auto *b = new mdsm_batch_s();
+  b->dtype = in_dtype;
+  b->beams.assign(beams, beams + nbeams);
+  b->maxshift.resize(nbeams);
+  b->h_tables.resize(nbeams);
+  b->d_tables.assign(nbeams, nullptr);
+  b->row_words.resize(nbeams);
+  const int nv = num_variants(in_dtype);
+  for (uint32_t i = 0; i < nbeams; ++i) {
+    int rc = build_delay_table(&beams[i], b->h_tables[i], &b->maxshift[i]);
+    if (rc != MDSM_OK) {
+      free_batch(b);
+      return rc;
+    }
+    b->row_words[i].assign(nv + 1, 0);
+    for (int v = 1; v <= nv; ++v) {
+      const TileShape sh = variant_shape(in_dtype, v);
+      b->row_words[i][v] = lds_row_words(
+          in_dtype, sh, tile_spread(b->h_tables[i].data(), beams[i].nchans, beams[i].ndms, sh.td()));
+    }
+    const size_t bytes = b->h_tables[i].size() * sizeof(int32_t);
+    if (hipMalloc(&b->d_tables[i], bytes) != hipSuccess ||
+        hipMemcpy(b->d_tables[i], b->h_tables[i].data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("delay table upload failed for beam %u", i);
+      free_batch(b);
+      return MDSM_ERR_NOMEM;
+    }
+  }
+  // largest beams (most channel-sample-trial adds) first: the queue tail
+  // then holds the cheapest items, which evens out the finish
+  b->order.resize(nbeams);
+  std::iota(b->order.begin(), b->order.end(), 0u);
+  std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {
+    const auto &a = beams[x], &c = beams[y];
+    return (double)a.nchans * a.ndms * a.nsamp > (double)c.nchans * c.ndms * c.nsamp;
+  });
+  if (hipMalloc(&b->d_desc, sizeof(BeamDesc) * nbeams) != hipSuccess ||
+      hipMalloc(&b->d_counter, sizeof(uint32_t)) != hipSuccess) {
+    set_error("batch descriptor allocation failed");
+    free_batch(b);
+    return MDSM_ERR_NOMEM;
+  }
+  *batch = b;
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_batch_destroy(mdsm_batch batch) {
+  if (batch) free_batch(batch);
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_batch_info(mdsm_batch batch, uint32_t beam, uint32_t *maxshift,
+                               uint64_t *min_ld) {
+  MDSM_REQUIRE(batch && beam < batch->beams.size(), MDSM_ERR_ARG, "bad beam index");
+  if (maxshift) *maxshift = batch->maxshift[beam];
+  if (min_ld) *min_ld = (uint64_t)batch->beams[beam].nsamp + batch->maxshift[beam];
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_batch_set_variant(mdsm_batch batch, int variant) {
+  MDSM_REQUIRE(batch, MDSM_ERR_ARG, "null batch");
+  MDSM_REQUIRE(variant >= 0 && variant <= num_variants(batch->dtype), MDSM_ERR_ARG,
+               "variant %d out of range", variant);
+  batch->variant = variant;
+  batch->bound = false;
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
+                               float *const *d_out) {
+  MDSM_REQUIRE(batch && d_in && ld && d_out, MDSM_ERR_ARG, "null argument");
+  const uint32_t nb = (uint32_t)batch->beams.size();
+  const int v = batch_variant(batch);
+  const TileShape sh = variant_shape(batch->dtype, v);
+  batch->h_desc.resize(nb);
+  uint64_t items = 0;
+  uint32_t maxrw = 0;
+  for (uint32_t k = 0; k < nb; ++k) {
+    const uint32_t i = batch->order[k];
+    const auto &s = batch->beams[i];
+    MDSM_REQUIRE(d_in[i] && d_out[i], MDSM_ERR_ARG, "null buffer for beam %u", i);
+    MDSM_REQUIRE(ld[i] >= (uint64_t)s.nsamp + batch->maxshift[i], MDSM_ERR_ARG,
+                 "beam %u: input row length below nsamp + maxshift", i);
+    BeamDesc &d = batch->h_desc[k];
+    d.in = d_in[i];
+    d.out = d_out[i];
+    d.table = batch->d_tables[i];
+    d.ld = ld[i];
+    d.nchans = s.nchans;
+    d.nsamp = s.nsamp;
+    d.ndms = s.ndms;
+    d.row_words = batch->row_words[i][v];
+    d.n_ttiles = (s.nsamp + sh.tt() - 1) / sh.tt();
+    d.n_dtiles = (s.ndms + sh.td() - 1) / sh.td();
+    d.item_begin = (uint32_t)items;
+    d.pad = 0;
+    items += (uint64_t)d.n_ttiles * d.n_dtiles;
+    maxrw = std::max(maxrw, d.row_words);
+  }
+  MDSM_REQUIRE(items < (1ull << 32) - 65536, MDSM_ERR_ARG, "too many work items");
+  MDSM_HIP(hipMemcpy(batch->d_desc, batch->h_desc.data(), sizeof(BeamDesc) * nb,
+                     hipMemcpyHostToDevice));
+  batch->total_items = (uint32_t)items;
+  batch->max_row_words = maxrw;
+  batch->bound = true;
+  batch->bound_variant = (uint32_t)v;
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_batch_launch(mdsm_batch batch, void *stream) {
+  MDSM_REQUIRE(batch && batch->bound, MDSM_ERR_STATE, "batch not bound to buffers");
+  hipStream_t s = as_stream(stream);
+  int rc = launch_zero_u32(batch->d_counter, 1, s);
+  if (rc != MDSM_OK) return rc;
+  return launch_batch_dedisperse(batch->dtype, (int)batch->bound_variant, batch->d_desc,
+                                 (uint32_t)batch->beams.size(), nullptr, batch->total_items,
+                                 batch->d_counter, batch->max_row_words, 0, s);
+}
+
+extern "C" int mdsm_batch_dedisperse(mdsm_batch batch, const void *const *d_in,
+                                     const uint64_t *ld, float *const *d_out, void *stream) {
+  int rc = mdsm_batch_bind(batch, d_in, ld, d_out);
+  if (rc != MDSM_OK) return rc;
+  return mdsm_batch_launch(batch, stream);
+}

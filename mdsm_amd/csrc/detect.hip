@@ -1,0 +1,453 @@
+// detect.hip -- bandpass normalisation, per-block statistics, boxcar
+// detection with wavefront compaction, median / detrend filters and the
+// device-side synthetic data generator (gfx950).
+//
+// Reductions use one fixed order everywhere: lane i of a 256-thread
+// workgroup sums elements i, i+256, ... sequentially in FP64, then a binary
+// tree p[i] += p[i+s] for s = 128..1.  With IEEE FP64 adds and no
+// contraction (-ffp-contract=off) the statistics are reproducible bit for bit
+// by a CPU restatement of the same order.  These kernels are HBM-bound.
+#include <hip/hip_fp16.h>
+
+#include "internal.h"
+
+namespace mdsm {
+namespace {
+
+constexpr int NT = 256;
+
+__device__ __forceinline__ void tree256(double *p1, double *p2) {
+  const int i = threadIdx.x;
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    __syncthreads();
+    if (i < s) {
+      p1[i] = p1[i] + p1[i + s];
+      p2[i] = p2[i] + p2[i + s];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void moments(double s1, double s2, uint64_t n, double *mean,
+                                        double *stdv) {
+  const double m = s1 / (double)n;
+  const double v = s2 / (double)n - m * m;
+  *mean = m;
+  *stdv = v > 0.0 ? sqrt(v) : 0.0;
+}
+
+// ----------------------------------------------------------- channel stats
+__global__ __launch_bounds__(NT) void channel_stats_kernel(const float *__restrict__ raw,
+                                                           uint64_t ld, uint64_t n,
+                                                           double *__restrict__ mean,
+                                                           double *__restrict__ stdv) {
+  __shared__ double p1[NT], p2[NT];
+  const uint32_t c = blockIdx.x;
+  const float *row = raw + (uint64_t)c * ld;
+  double a = 0.0, b = 0.0;
+  for (uint64_t j = threadIdx.x; j < n; j += NT) {
+    const double v = (double)row[j];
+    a = a + v;
+    b = b + v * v;
+  }
+  p1[threadIdx.x] = a;
+  p2[threadIdx.x] = b;
+  tree256(p1, p2);
+  if (threadIdx.x == 0) moments(p1[0], p2[0], n, &mean[c], &stdv[c]);
+}
+
+// ------------------------------------------------------------- normalise
+__device__ __forceinline__ uint8_t quant_u8(double v) {
+  v = v < -4.0 ? -4.0 : v;
+  v = v > 4.0 ? 4.0 : v;
+  return (uint8_t)floor((v + 4.0) * 31.875 + 0.5);
+}
+
+template <int DT>
+__global__ __launch_bounds__(NT) void normalise_kernel(const float *__restrict__ raw, uint64_t ld,
+                                                       uint64_t n, const double *__restrict__ mean,
+                                                       const double *__restrict__ stdv,
+                                                       void *__restrict__ out, uint64_t ldo) {
+  const uint32_t c = blockIdx.y;
+  const double m = mean[c];
+  const double s = stdv[c];
+  const double inv = s > 0.0 ? 1.0 / s : 0.0;
+  const float *row = raw + (uint64_t)c * ld;
+  for (uint64_t t = (uint64_t)blockIdx.x * NT + threadIdx.x; t < n; t += (uint64_t)gridDim.x * NT) {
+    const double v = ((double)row[t] - m) * inv;
+    if (DT == MDSM_F32) reinterpret_cast<float *>(out)[(uint64_t)c * ldo + t] = (float)v;
+    else if (DT == MDSM_F16)
+      reinterpret_cast<__half *>(out)[(uint64_t)c * ldo + t] = __float2half_rn((float)v);
+    else reinterpret_cast<uint8_t *>(out)[(uint64_t)c * ldo + t] = quant_u8(v);
+  }
+}
+
+// ------------------------------------------------------------ block stats
+__global__ __launch_bounds__(NT) void block_stats_kernel(const float *__restrict__ x,
+                                                         uint32_t nsamp, uint32_t L, uint32_t nb,
+                                                         double *__restrict__ mean,
+                                                         double *__restrict__ stdv) {
+  __shared__ double p1[NT], p2[NT];
+  const uint32_t b = blockIdx.x, d = blockIdx.y;
+  const uint64_t t0 = (uint64_t)b * L;
+  const uint32_t n = (uint32_t)min((uint64_t)L, (uint64_t)nsamp - t0);
+  const float *row = x + (uint64_t)d * nsamp + t0;
+  double a = 0.0, q = 0.0;
+  for (uint32_t j = threadIdx.x; j < n; j += NT) {
+    const double v = (double)row[j];
+    a = a + v;
+    q = q + v * v;
+  }
+  p1[threadIdx.x] = a;
+  p2[threadIdx.x] = q;
+  tree256(p1, p2);
+  if (threadIdx.x == 0) moments(p1[0], p2[0], n, &mean[(uint64_t)d * nb + b], &stdv[(uint64_t)d * nb + b]);
+}
+
+// ----------------------------------------------------------------- detect
+// Tile of DT_T samples of one DM row, staged in LDS as FP64 with a 64-sample
+// halo; the dyadic boxcar levels S_k(t) = S_{k-1}(t) + S_{k-1}(t + 2^(k-1))
+// are built in place, one barrier pair per level.
+constexpr int DT_T = 1024;
+constexpr int DT_HALO = 64;
+constexpr int DT_Q = DT_T / NT;
+
+__constant__ double kSqrtW[7] = {1.0, 1.4142135623730951, 2.0, 2.8284271247461903,
+                                 4.0, 5.656854249492381, 8.0};
+
+__device__ __forceinline__ void emit(bool hit, const mdsm_cand_t &cand, mdsm_cand_t *cands,
+                                     uint32_t cap, uint32_t *count) {
+  // wave-level compaction: one atomic per wave, lane prefix from the ballot
+  const unsigned long long mask = __ballot(hit);
+  if (mask == 0ull) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+  base = __shfl(base, (int)leader);
+  const unsigned long long lt = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+  const uint32_t idx = base + (uint32_t)__popcll(lt);
+  if (hit && idx < cap) cands[idx] = cand;
+}
+
+__global__ __launch_bounds__(NT) void detect_kernel(const float *__restrict__ x, uint32_t nsamp,
+                                                    uint32_t L, uint32_t nb,
+                                                    const double *__restrict__ mean,
+                                                    const double *__restrict__ stdv, int nwidths,
+                                                    double thresh, mdsm_cand_t *__restrict__ cands,
+                                                    uint32_t cap, uint32_t *__restrict__ count) {
+  __shared__ double lev[DT_T + DT_HALO];
+  const uint32_t d = blockIdx.y;
+  const uint32_t t0 = blockIdx.x * DT_T;
+  const float *row = x + (uint64_t)d * nsamp;
+  for (uint32_t i = threadIdx.x; i < DT_T + DT_HALO; i += NT) {
+    const uint32_t t = t0 + i;
+    lev[i] = t < nsamp ? (double)row[t] : 0.0;
+  }
+  double best[DT_Q];
+  int bk[DT_Q];
+#pragma unroll
+  for (int q = 0; q < DT_Q; ++q) { best[q] = -INFINITY; bk[q] = -1; }
+  for (int k = 0; k < nwidths; ++k) {
+    const uint32_t w = 1u << k;
+    if (k > 0) {
+      const uint32_t h = w >> 1;
+      double v[(DT_T + DT_HALO + NT - 1) / NT];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < (DT_T + DT_HALO + NT - 1) / NT; ++q) {
+        const uint32_t i = threadIdx.x + q * NT;
+        v[q] = (i + h < DT_T + DT_HALO) ? lev[i] + lev[i + h] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < (DT_T + DT_HALO + NT - 1) / NT; ++q) {
+        const uint32_t i = threadIdx.x + q * NT;
+        if (i < DT_T + DT_HALO) lev[i] = v[q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < DT_Q; ++q) {
+      const uint32_t i = threadIdx.x + q * NT;
+      const uint32_t t = t0 + i;
+      if ((uint64_t)t + w <= nsamp) {
+        const uint32_t b = t / L;
+        const double m = mean[(uint64_t)d * nb + b];
+        const double s = stdv[(uint64_t)d * nb + b];
+        if (s > 0.0) {
+          const double snr = (lev[i] - (double)w * m) / (s * kSqrtW[k]);
+          if (snr > best[q]) { best[q] = snr; bk[q] = k; }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < DT_Q; ++q) {
+    const uint32_t t = t0 + threadIdx.x + q * NT;
+    const bool hit = bk[q] >= 0 && best[q] >= thresh;
+    mdsm_cand_t cand{(int32_t)d, (int32_t)t, bk[q], (float)best[q]};
+    emit(hit, cand, cands, cap, count);
+  }
+}
+
+// ------------------------------------------------- global sums / threshold
+__global__ __launch_bounds__(NT) void global_sums_kernel(const float *__restrict__ x, uint64_t n,
+                                                         double *__restrict__ partials) {
+  __shared__ double p1[NT], p2[NT];
+  double a = 0.0, b = 0.0;
+  for (uint64_t j = (uint64_t)blockIdx.x * NT + threadIdx.x; j < n; j += (uint64_t)gridDim.x * NT) {
+    const double v = (double)x[j];
+    a = a + v;
+    b = b + v * v;
+  }
+  p1[threadIdx.x] = a;
+  p2[threadIdx.x] = b;
+  tree256(p1, p2);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = p1[0];
+    partials[2 * blockIdx.x + 1] = p2[0];
+  }
+}
+
+__global__ __launch_bounds__(NT) void threshold_global_kernel(const float *__restrict__ x,
+                                                              uint32_t nsamp, uint64_t n,
+                                                              double mean, double rms,
+                                                              float thresh,
+                                                              mdsm_cand_t *__restrict__ cands,
+                                                              uint32_t cap,
+                                                              uint32_t *__restrict__ count) {
+  const double lim = rms * (double)thresh;
+  for (uint64_t base = (uint64_t)blockIdx.x * NT; base < n; base += (uint64_t)gridDim.x * NT) {
+    const uint64_t j = base + threadIdx.x;
+    const bool in = j < n;
+    const float v = in ? x[j] : 0.0f;
+    const bool hit = in && ((double)v - mean >= lim);
+    mdsm_cand_t cand{(int32_t)(j / nsamp), (int32_t)(j % nsamp), 0,
+                     (float)(((double)v - mean) / rms)};
+    emit(hit, cand, cands, cap, count);
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ----------------------------------------------------------------- median
+__global__ __launch_bounds__(NT) void median5_kernel(const float *__restrict__ x, uint32_t nsamp,
+                                                     float *__restrict__ out) {
+  const uint32_t d = blockIdx.y;
+  const uint32_t t = blockIdx.x * NT + threadIdx.x;
+  if (t >= nsamp) return;
+  const float *r = x + (uint64_t)d * nsamp;
+  float v[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    int i = (int)t + k - 2;
+    i = i < 0 ? 0 : (i >= (int)nsamp ? (int)nsamp - 1 : i);
+    v[k] = r[i];
+  }
+#pragma unroll
+  for (int i = 1; i < 5; ++i)
+#pragma unroll
+    for (int j = i; j > 0; --j)
+      if (v[j] < v[j - 1]) { const float s = v[j]; v[j] = v[j - 1]; v[j - 1] = s; }
+  out[(uint64_t)d * nsamp + t] = v[2];
+}
+
+// ---------------------------------------------------------------- detrend
+__global__ __launch_bounds__(NT) void detrend_kernel(float *__restrict__ x, uint32_t L) {
+  __shared__ double p1[NT], p2[NT];
+  float *y = x + (uint64_t)blockIdx.x * L;
+  const double half = -(double)L / 2.0 + 0.5;
+  double sy = 0.0, sxy = 0.0;
+  for (uint32_t i = threadIdx.x; i < L; i += NT) {
+    const double xi = half + (double)i;
+    sy = sy + (double)y[i];
+    sxy = sxy + xi * (double)y[i];
+  }
+  p1[threadIdx.x] = sy;
+  p2[threadIdx.x] = sxy;
+  tree256(p1, p2);
+  // sum of x_i^2 for x_i = i - (L-1)/2 is (L^3 - L)/12
+  const double sxx = ((double)L * (double)L * (double)L - (double)L) / 12.0;
+  const double a = p1[0] / (double)L, b = p2[0] / sxx;
+  __syncthreads();
+  double ss = 0.0;
+  for (uint32_t i = threadIdx.x; i < L; i += NT) {
+    const double r = (double)y[i] - (a + b * (half + (double)i));
+    ss = ss + r * r;
+  }
+  p1[threadIdx.x] = ss;
+  p2[threadIdx.x] = 0.0;
+  tree256(p1, p2);
+  const double sd = sqrt(p1[0] / (double)L);
+  for (uint32_t i = threadIdx.x; i < L; i += NT) {
+    const double r = (double)y[i] - (a + b * (half + (double)i));
+    y[i] = (float)(r / sd);
+  }
+}
+
+// -------------------------------------------------------------- generator
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    const uint32_t n3 = (uint32_t)p0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ __forceinline__ double u24(uint32_t x) {
+  return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0);
+}
+
+__global__ __launch_bounds__(NT) void generate_kernel(uint64_t seed, int64_t t0, uint64_t n,
+                                                      const float *__restrict__ gain,
+                                                      const float *__restrict__ offset,
+                                                      uint32_t npulses, uint32_t nchans,
+                                                      const int64_t *__restrict__ p_start,
+                                                      const int32_t *__restrict__ p_width,
+                                                      const float *__restrict__ p_amp,
+                                                      const int64_t *__restrict__ p_delay,
+                                                      float *__restrict__ raw, uint64_t ld) {
+  const uint32_t c = blockIdx.y;
+  const float g = gain[c], o = offset[c];
+  float *row = raw + (uint64_t)c * ld;
+  // one thread per quad of samples q = floor(t / 4)
+  const int64_t q0 = t0 >> 2;
+  const int64_t qlast = (t0 + (int64_t)n - 1) >> 2;
+  for (int64_t q = q0 + (int64_t)blockIdx.x * NT + threadIdx.x; q <= qlast;
+       q += (int64_t)gridDim.x * NT) {
+    uint32_t ctr[4] = {(uint32_t)q, (uint32_t)((uint64_t)q >> 32), c, 0u};
+    philox(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double two_pi = 6.283185307179586;
+    const double r0 = sqrt(-2.0 * log(u24(ctr[0]))), a0 = two_pi * u24(ctr[1]);
+    const double r1 = sqrt(-2.0 * log(u24(ctr[2]))), a1 = two_pi * u24(ctr[3]);
+    float z[4] = {(float)(r0 * cos(a0)), (float)(r0 * sin(a0)), (float)(r1 * cos(a1)),
+                  (float)(r1 * sin(a1))};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t t = q * 4 + k;
+      if (t < t0 || t >= t0 + (int64_t)n) continue;
+      float v = z[k];
+      for (uint32_t p = 0; p < npulses; ++p) {
+        const int64_t s = p_start[p] + p_delay[(uint64_t)p * nchans + c];
+        if (t >= s && t < s + p_width[p]) v = v + p_amp[p];
+      }
+      row[t - t0] = g * v + o;
+    }
+  }
+}
+
+int grid_for(uint64_t work, int per_block, int cap) {
+  uint64_t g = (work + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > (uint64_t)cap) g = cap;
+  return (int)g;
+}
+
+}  // namespace
+
+int launch_channel_stats(const float *raw, uint64_t ld, uint32_t nchans, uint64_t n, double *mean,
+                         double *stdv, hipStream_t s) {
+  hipLaunchKernelGGL(channel_stats_kernel, dim3(nchans), dim3(NT), 0, s, raw, ld, n, mean, stdv);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_normalise(const float *raw, uint64_t ld, uint32_t nchans, uint64_t n,
+                     const double *mean, const double *stdv, void *out, int out_dtype,
+                     uint64_t ldo, hipStream_t s) {
+  const dim3 grid(grid_for(n, NT * 8, 4096), nchans);
+  switch (out_dtype) {
+    case MDSM_F32:
+      hipLaunchKernelGGL(normalise_kernel<MDSM_F32>, grid, dim3(NT), 0, s, raw, ld, n, mean, stdv, out, ldo);
+      break;
+    case MDSM_F16:
+      hipLaunchKernelGGL(normalise_kernel<MDSM_F16>, grid, dim3(NT), 0, s, raw, ld, n, mean, stdv, out, ldo);
+      break;
+    case MDSM_U8:
+      hipLaunchKernelGGL(normalise_kernel<MDSM_U8>, grid, dim3(NT), 0, s, raw, ld, n, mean, stdv, out, ldo);
+      break;
+    default:
+      set_error("unknown output dtype %d", out_dtype);
+      return MDSM_ERR_ARG;
+  }
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_block_stats(const float *x, uint32_t nrows, uint32_t nsamp, uint32_t L, double *mean,
+                       double *stdv, hipStream_t s) {
+  const uint32_t nb = (nsamp + L - 1) / L;
+  hipLaunchKernelGGL(block_stats_kernel, dim3(nb, nrows), dim3(NT), 0, s, x, nsamp, L, nb, mean, stdv);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_detect(const float *x, uint32_t nrows, uint32_t nsamp, uint32_t L, const double *mean,
+                  const double *stdv, int nwidths, double thresh, mdsm_cand_t *cands, uint32_t cap,
+                  uint32_t *count, hipStream_t s) {
+  const uint32_t nb = (nsamp + L - 1) / L;
+  const uint32_t ntile = (nsamp + DT_T - 1) / DT_T;
+  hipLaunchKernelGGL(detect_kernel, dim3(ntile, nrows), dim3(NT), 0, s, x, nsamp, L, nb, mean, stdv,
+                     nwidths, thresh, cands, cap, count);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_global_sums(const float *x, uint64_t n, double *partials, uint32_t nparts,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(global_sums_kernel, dim3(nparts), dim3(NT), 0, s, x, n, partials);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_threshold_global(const float *x, uint32_t nrows, uint32_t nsamp, double mean,
+                            double rms, float thresh, mdsm_cand_t *cands, uint32_t cap,
+                            uint32_t *count, hipStream_t s) {
+  const uint64_t n = (uint64_t)nrows * nsamp;
+  hipLaunchKernelGGL(threshold_global_kernel, dim3(grid_for(n, NT * 16, 65536)), dim3(NT), 0, s, x,
+                     nsamp, n, mean, rms, thresh, cands, cap, count);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_median5(const float *x, uint32_t nrows, uint32_t nsamp, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(median5_kernel, dim3((nsamp + NT - 1) / NT, nrows), dim3(NT), 0, s, x, nsamp, out);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_detrend(float *x, uint64_t nseg, uint32_t L, hipStream_t s) {
+  hipLaunchKernelGGL(detrend_kernel, dim3((uint32_t)nseg), dim3(NT), 0, s, x, L);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_generate(uint64_t seed, uint32_t nchans, int64_t t0, uint64_t n, const float *gain,
+                    const float *offset, uint32_t npulses, const int64_t *p_start,
+                    const int32_t *p_width, const float *p_amp, const int64_t *p_delay, float *raw,
+                    uint64_t ld, hipStream_t s) {
+  const dim3 grid(grid_for(n / 4 + 2, NT * 4, 4096), nchans);
+  hipLaunchKernelGGL(generate_kernel, grid, dim3(NT), 0, s, seed, t0, n, gain, offset, npulses,
+                     nchans, p_start, p_width, p_amp, p_delay, raw, ld);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+}  // namespace mdsm

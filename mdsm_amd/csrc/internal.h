@@ -1,0 +1,104 @@
+// internal.h -- shared declarations of libmdsm_amd (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "mdsm_abi.h"
+
+namespace mdsm {
+
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define MDSM_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      ::mdsm::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                       \
+      return MDSM_ERR_HIP;                                                               \
+    }                                                                                    \
+  } while (0)
+
+#define MDSM_REQUIRE(cond, code, ...) \
+  do {                                \
+    if (!(cond)) {                    \
+      ::mdsm::set_error(__VA_ARGS__); \
+      return code;                    \
+    }                                 \
+  } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------- tiles
+// A dedispersion tile variant: every wave owns MW consecutive DM trials and
+// K time samples per lane (lane + 64*j), a workgroup of 4 waves owns
+// TD = 4*MW DM trials x TT = 64*K samples, and CB channels are staged in LDS
+// per barrier.
+struct TileShape {
+  int mw, k, cb;
+  int td() const { return 4 * mw; }
+  int tt() const { return 64 * k; }
+};
+
+int num_variants(int dtype);
+TileShape variant_shape(int dtype, int variant);
+
+// Largest delay spread inside any DM tile of width td, over all channels:
+// max_c max_{tile} D[c][min(d0+td,M)-1] - D[c][d0].
+uint32_t tile_spread(const int32_t *table, uint32_t nchans, uint32_t ndms, int td);
+
+// LDS words per staged channel row for a shape (samples + spread, padded).
+uint32_t lds_row_words(int dtype, const TileShape &sh, uint32_t spread);
+
+// Validates survey parameters and fills the host delay table.
+int build_delay_table(const mdsm_survey_t *s, std::vector<int32_t> &table, uint32_t *maxshift);
+
+// ---------------------------------------------------------------- kernels
+int launch_dedisperse(int dtype, int variant, const void *in, uint64_t ld, const int32_t *table,
+                      uint32_t nchans, uint32_t nsamp, uint32_t ndms, uint32_t row_words,
+                      float *out, hipStream_t stream);
+
+// Per-beam descriptor of the persistent batched dedisperser.
+struct BeamDesc {
+  const void *in;
+  float *out;
+  const int32_t *table;
+  uint64_t ld;
+  uint32_t nchans, nsamp, ndms, row_words;
+  uint32_t n_ttiles, n_dtiles;
+  uint32_t item_begin;  // first work item of this beam in the queue order
+  uint32_t pad;
+};
+
+int launch_batch_dedisperse(int dtype, int variant, const BeamDesc *d_beams, uint32_t nbeams,
+                            const uint32_t *d_order, uint32_t total_items, uint32_t *d_counter,
+                            uint32_t max_row_words, int grid, hipStream_t stream);
+size_t batch_lds_bytes(int dtype, int variant, uint32_t max_row_words);
+
+int launch_channel_stats(const float *raw, uint64_t ld, uint32_t nchans, uint64_t n, double *mean,
+                         double *stdv, hipStream_t s);
+int launch_normalise(const float *raw, uint64_t ld, uint32_t nchans, uint64_t n,
+                     const double *mean, const double *stdv, void *out, int out_dtype,
+                     uint64_t ldo, hipStream_t s);
+int launch_block_stats(const float *x, uint32_t nrows, uint32_t nsamp, uint32_t L, double *mean,
+                       double *stdv, hipStream_t s);
+int launch_detect(const float *x, uint32_t nrows, uint32_t nsamp, uint32_t L, const double *mean,
+                  const double *stdv, int nwidths, double thresh, mdsm_cand_t *cands, uint32_t cap,
+                  uint32_t *count, hipStream_t s);
+int launch_global_sums(const float *x, uint64_t n, double *partials, uint32_t nparts,
+                       hipStream_t s);
+int launch_threshold_global(const float *x, uint32_t nrows, uint32_t nsamp, double mean,
+                            double rms, float thresh, mdsm_cand_t *cands, uint32_t cap,
+                            uint32_t *count, hipStream_t s);
+int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t s);
+int launch_median5(const float *x, uint32_t nrows, uint32_t nsamp, float *out, hipStream_t s);
+int launch_detrend(float *x, uint64_t nseg, uint32_t L, hipStream_t s);
+int launch_generate(uint64_t seed, uint32_t nchans, int64_t t0, uint64_t n, const float *gain,
+                    const float *offset, uint32_t npulses, const int64_t *p_start,
+                    const int32_t *p_width, const float *p_amp, const int64_t *p_delay, float *raw,
+                    uint64_t ld, hipStream_t s);
+
+}  // namespace mdsm

@@ -1,0 +1,101 @@
+"""CPU-only checks of the product library: it loads, exports every symbol the
+C header declares, and its host-side logic (delay tables, survey validation,
+clustering) matches the oracle.  No GPU compute is called here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mdsm_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mdsm_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = _declared("mdsm_abi.h")
+    assert len(names) > 40
+    lib = abi.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("mode", [abi.DELAY_NORTHSTAR, abi.DELAY_REFERENCE])
+@pytest.mark.parametrize("cfg", [
+    (418.0, -0.078125, 256, 51.2e-6, 0.0, 0.1, 256),
+    (1500.0, -0.29296875, 1024, 64e-6, 0.0, 0.2, 4096),
+    (1400.0, -0.5, 512, 100e-6, 3.0, 0.37, 999),
+])
+def test_delay_table_matches_oracle(cfg, mode):
+    fch1, foff, C, ts, lo, st, M = cfg
+    t, ms = abi.delay_table(abi.Survey.make(fch1, foff, C, ts, lo, st, M, 1024, mode))
+    ref = (O.delay_table if mode == abi.DELAY_NORTHSTAR else O.delay_table_ref)(*cfg)
+    assert np.array_equal(t, ref)
+    assert ms == ref.max()
+
+
+def test_survey_validation_errors():
+    bad = [
+        abi.Survey.make(1500.0, +0.3, 64, 64e-6, 0.0, 1.0, 8, 1024),   # ascending band
+        abi.Survey.make(1500.0, -0.3, 0, 64e-6, 0.0, 1.0, 8, 1024),    # no channels
+        abi.Survey.make(1500.0, -0.3, 64, 0.0, 0.0, 1.0, 8, 1024),     # tsamp 0
+        abi.Survey.make(100.0, -1.0, 128, 64e-6, 0.0, 1.0, 8, 1024),   # negative frequency
+        abi.Survey.make(1500.0, -0.3, 64, 64e-6, -1.0, 1.0, 8, 1024),  # negative DM
+    ]
+    for s in bad:
+        with pytest.raises(abi.MdsmError):
+            abi.delay_table(s)
+
+
+def _cands(d, t, snr=None):
+    c = np.zeros(len(d), abi.CAND_DTYPE)
+    c["dm_idx"] = d
+    c["sample"] = t
+    c["snr"] = np.arange(len(d), dtype=np.float32) if snr is None else snr
+    return c
+
+
+def test_dbscan2d_matches_oracle_and_sklearn():
+    from sklearn.cluster import DBSCAN
+    rng = np.random.default_rng(7)
+    pts = np.concatenate([rng.integers(0, 300, (400, 2)), rng.integers(0, 8, (80, 2)) + 100,
+                          np.stack([np.arange(40) * 3, np.arange(40)], 1)]).astype(np.int32)
+    pts = np.unique(pts, axis=0)
+    c = _cands(pts[:, 0], pts[:, 1], rng.random(len(pts)).astype(np.float32))
+    abi.sort_candidates(c)
+    lab, n = abi.dbscan2d(c, 3, 4)
+    olab, on = O.dbscan2d(c["dm_idx"], c["sample"], 3, 4)
+    assert n == on and np.array_equal(lab, olab)
+    sk = DBSCAN(eps=3, min_samples=4, metric="chebyshev").fit(
+        np.stack([c["dm_idx"], c["sample"]], 1)).labels_
+    assert np.array_equal(lab, sk)
+    peaks, sizes = abi.cluster_peaks(c, lab, n)
+    for k in range(n):
+        m = lab == k
+        assert sizes[k] == m.sum()
+        assert peaks[k]["snr"] == c["snr"][m].max()
+
+
+def test_dbscan2d_rejects_unsorted():
+    c = _cands([5, 1], [0, 0])
+    with pytest.raises(abi.MdsmError):
+        abi.dbscan2d(c)
+
+
+def test_dbscan_ref_matches_oracle():
+    rng = np.random.default_rng(8)
+    n = 300
+    tm = np.round(rng.random(n) * 2.0, 3)
+    dm = np.round(rng.random(n) * 10.0, 1)
+    snr = 6 + rng.random(n) * 3
+    for mp in (1, 2, 3, 5):
+        lab, k = abi.dbscan_ref(tm, dm, snr, 0.05, 1.0, 1.5, mp)
+        olab, ok = O.dbscan_ref(tm, dm, snr, 0.05, 1.0, 1.5, mp)
+        assert k == ok and np.array_equal(lab, olab)

@@ -1,0 +1,125 @@
+"""GPU parity of the dedispersion kernels against the CPU oracle.
+
+Bar: bit-exact against the oracle's FP32 channel-order sum (the kernels add
+in channel order 0..C-1, one rounding per add) and against the oracle's
+exact-integer U8 sum; max error against the FP64 oracle stated per case
+(tolerance: max |y - y64| <= 1e-5 * rms(y64) * sqrt(C) for FP32 inputs).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mdsm_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+CFG0 = dict(fch1=418.0, foff=-0.078125, nchans=256, tsamp=51.2e-6, lowdm=0.0, dmstep=0.1, ndms=256)
+
+
+def _say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def _survey(nsamp, mode=abi.DELAY_NORTHSTAR, **kw):
+    p = dict(CFG0)
+    p.update(kw)
+    return abi.Survey.make(p["fch1"], p["foff"], p["nchans"], p["tsamp"], p["lowdm"], p["dmstep"],
+                           p["ndms"], nsamp, mode)
+
+
+def _table(s):
+    t, _ = abi.delay_table(s)
+    return t
+
+
+def _err_stats(y, y64):
+    e = np.abs(y.astype(np.float64) - y64)
+    return e.max(), e.max() / np.sqrt((y64 * y64).mean())
+
+
+@pytest.mark.parametrize("variant", list(range(1, abi.tile_variants() + 1)))
+def test_f32_cfg0_bitexact(variant):
+    s = _survey(65536)
+    t = _table(s)
+    S = int(t.max())
+    _say(f"[f32 cfg0 variant {variant}] C=256 N=65536 M=256 S={S}")
+    x = O.generate(1, 256, s.fch1, s.foff, s.tsamp, 0, 65536 + S)
+    y = abi.dedisperse_host(s, x, abi.F32, variant)
+    ref = O.dedisperse_f32seq(x, t, 65536)
+    assert np.array_equal(y, ref)
+    if variant == 1:
+        y64 = O.dedisperse_f64(x, t, 65536)
+        mx, rel = _err_stats(y, y64)
+        _say(f"    max|y-y64| = {mx:.3e}, / rms = {rel:.3e}")
+        assert rel < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(37, 45, 1000, 2.5, 1.7), (1, 1, 1, 0.0, 1.0), (130, 129, 777, 0.0, 9.0),
+                                   (64, 300, 4096, 10.0, 0.05)])
+def test_f32_ragged_shapes(shape):
+    C, M, N, lo, step = shape
+    s = _survey(N, nchans=C, ndms=M, lowdm=lo, dmstep=step, foff=-0.078125 * 256 / max(C, 1) / 4)
+    t = _table(s)
+    S = int(t.max())
+    _say(f"[f32 ragged] C={C} M={M} N={N} S={S}")
+    x = np.random.default_rng(C * 1000 + M).standard_normal((C, N + S + 3)).astype(np.float32)
+    ref = O.dedisperse_f32seq(x, t, N)
+    for v in range(1, abi.tile_variants() + 1):
+        assert np.array_equal(abi.dedisperse_host(s, x, abi.F32, v), ref), v
+
+
+def test_known_answer_channel_index():
+    s = _survey(4096)
+    t = _table(s)
+    S = int(t.max())
+    x = np.repeat(np.arange(256, dtype=np.float32)[:, None], 4096 + S, axis=1)
+    y = abi.dedisperse_host(s, x)
+    assert (y == 256 * 255 / 2).all()
+
+
+def test_f16_u8_bitexact_and_error():
+    s = _survey(16384)
+    t = _table(s)
+    S = int(t.max())
+    _say(f"[f16/u8] C=256 N=16384 M=256 S={S}")
+    x = O.generate(5, 256, s.fch1, s.foff, s.tsamp, 0, 16384 + S)
+    h = O.f32_to_f16(x)
+    y16 = abi.dedisperse_host(s, h, abi.F16)
+    assert np.array_equal(y16, O.dedisperse_f16(h, t, 16384))
+    m, sd = O.channel_stats(x)
+    u = O.normalise(x, m, sd, "u8")
+    y8 = abi.dedisperse_host(s, u, abi.U8)
+    assert np.array_equal(y8, O.dedisperse_u8(u, t, 16384))
+    # errors against FP64 of the unquantised input, reported
+    y64 = O.dedisperse_f64(x, t, 16384)
+    mx, rel = _err_stats(y16, y64)
+    _say(f"    f16 storage: max|y-y64|/rms = {rel:.3e}")
+    assert rel < 2e-3
+
+
+def test_reference_delay_mode():
+    s = _survey(8192, mode=abi.DELAY_REFERENCE)
+    t = O.delay_table_ref(s.fch1, s.foff, 256, s.tsamp, s.lowdm, s.dmstep, 256)
+    S = int(t.max())
+    x = np.random.default_rng(3).standard_normal((256, 8192 + S)).astype(np.float32)
+    assert np.array_equal(abi.dedisperse_host(s, x), O.dedisperse_f32seq(x, t, 8192))
+
+
+def test_rejects_short_rows_and_autotune():
+    s = _survey(8192)
+    plan = abi.Plan(s)
+    din = abi.DeviceBuffer(256 * plan.min_ld * 4)
+    dout = abi.DeviceBuffer(256 * 8192 * 4)
+    with pytest.raises(abi.MdsmError):
+        plan.dedisperse(din.ptr, plan.min_ld - 1, dout.ptr)
+    x = np.random.default_rng(4).standard_normal((256, plan.min_ld)).astype(np.float32)
+    din.upload(x)
+    v, ms = plan.autotune(din.ptr, plan.min_ld, dout.ptr)
+    _say(f"[autotune] best variant {v} {ms:.3f} ms")
+    assert plan.variant == v
+    plan.dedisperse(din.ptr, plan.min_ld, dout.ptr)
+    abi.synchronize()
+    t = _table(s)
+    assert np.array_equal(dout.download(np.float32, (256, 8192)), O.dedisperse_f32seq(x, t, 8192))

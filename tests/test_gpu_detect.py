@@ -1,0 +1,168 @@
+"""GPU parity of normalisation, statistics, detection, filters, generator and
+the end-to-end single-beam search against the CPU oracle.
+
+Bars: per-channel and per-block FP64 statistics, normalised F32/F16/U8
+arrays and candidate lists (dm_idx, sample, width) are bit-exact against the
+oracle fed the same inputs (the kernels use the oracle's reduction order and
+FP64 arithmetic without contraction).  Against the FP64 end-to-end oracle the
+clustered candidate peaks must be identical.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mdsm_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def _cands_tuple(c):
+    return list(zip(c["dm_idx"].tolist(), c["sample"].tolist(), c["width_log2"].tolist()))
+
+
+def test_channel_stats_and_normalise_bitexact():
+    C, n, ld = 64, 50001, 50011
+    g, o = O.channel_bandpass(3, C)
+    raw = O.generate(3, C, 1500.0, -0.29296875, 64e-6, 0, n, g, o, ld=ld)
+    _say(f"[normalise] C={C} n={n}")
+    d_raw = abi.DeviceBuffer.from_numpy(raw)
+    d_m, d_s = abi.DeviceBuffer(C * 8), abi.DeviceBuffer(C * 8)
+    abi.channel_stats(d_raw.ptr, ld, C, n, d_m.ptr, d_s.ptr)
+    m, s = d_m.download(np.float64, (C,)), d_s.download(np.float64, (C,))
+    om, os_ = O.channel_stats(raw, n)
+    assert np.array_equal(m, om) and np.array_equal(s, os_)
+    for dt, name, npdt in [(abi.F32, "f32", np.float32), (abi.F16, "f16", np.uint16), (abi.U8, "u8", np.uint8)]:
+        ldo = n + 7
+        d_o = abi.DeviceBuffer(C * ldo * np.dtype(npdt).itemsize)
+        abi.normalise(d_raw.ptr, ld, C, n, d_m.ptr, d_s.ptr, d_o.ptr, dt, ldo)
+        got = d_o.download(npdt, (C, ldo))[:, :n]
+        assert np.array_equal(got, O.normalise(raw, om, os_, name, n)), name
+
+
+def _dedispersed(seed, C, M, N, pulses=()):
+    s = abi.Survey.make(418.0, -0.078125 * 256 / C, C, 51.2e-6, 0.0, 0.1, M, N)
+    t, S = abi.delay_table(s)
+    x = O.generate(seed, C, s.fch1, s.foff, s.tsamp, 0, N + S, pulses=pulses)
+    return s, t, x, abi.dedisperse_host(s, x)
+
+
+@pytest.mark.parametrize("L", [16384, 4096, 1000])
+def test_block_stats_and_detect_bitexact(L):
+    s, t, x, y = _dedispersed(11, 128, 96, 40000, pulses=[(3.0, 5000, 16, 0.6), (7.7, 20000, 3, 1.5)])
+    M, N = 96, 40000
+    _say(f"[detect] M={M} N={N} L={L}")
+    d_y = abi.DeviceBuffer.from_numpy(y)
+    nb = (N + L - 1) // L
+    d_m, d_s = abi.DeviceBuffer(M * nb * 8), abi.DeviceBuffer(M * nb * 8)
+    abi.block_stats(d_y.ptr, M, N, L, d_m.ptr, d_s.ptr)
+    m, sd = d_m.download(np.float64, (M, nb)), d_s.download(np.float64, (M, nb))
+    om, os_ = O.block_stats(y.astype(np.float64), L)
+    assert np.array_equal(m, om) and np.array_equal(sd, os_)
+    cap = 1 << 16
+    d_c, d_n = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+    for thr in (6.0, 4.5):
+        abi.detect(d_y.ptr, M, N, L, d_m.ptr, d_s.ptr, 7, thr, d_c.ptr, cap, d_n.ptr)
+        c = abi.read_candidates(d_c, d_n, cap)
+        od, ot, ok, osnr, on = O.detect(y.astype(np.float64), L, 7, thr, stats=(om, os_))
+        assert len(c) == on > 0
+        assert _cands_tuple(c) == list(zip(od.tolist(), ot.tolist(), ok.tolist()))
+        assert np.array_equal(c["snr"], osnr)
+
+
+def test_detect_overflow_is_reported():
+    s, t, x, y = _dedispersed(12, 64, 32, 8192)
+    d_y = abi.DeviceBuffer.from_numpy(y)
+    d_m, d_s = abi.DeviceBuffer(32 * 8), abi.DeviceBuffer(32 * 8)
+    abi.block_stats(d_y.ptr, 32, 8192, 8192, d_m.ptr, d_s.ptr)
+    d_c, d_n = abi.DeviceBuffer(16 * 16), abi.DeviceBuffer(4)
+    abi.detect(d_y.ptr, 32, 8192, 8192, d_m.ptr, d_s.ptr, 7, 1.0, d_c.ptr, 16, d_n.ptr)
+    with pytest.raises(abi.MdsmError):
+        abi.read_candidates(d_c, d_n, 16)
+
+
+def _gpu_search(s, x, L=16384, thr=6.0):
+    y = abi.dedisperse_host(s, x)
+    M, N = s.ndms, s.nsamp
+    nb = (N + L - 1) // L
+    d_y = abi.DeviceBuffer.from_numpy(y)
+    d_m, d_s = abi.DeviceBuffer(M * nb * 8), abi.DeviceBuffer(M * nb * 8)
+    abi.block_stats(d_y.ptr, M, N, L, d_m.ptr, d_s.ptr)
+    cap = 1 << 20
+    d_c, d_n = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+    abi.detect(d_y.ptr, M, N, L, d_m.ptr, d_s.ptr, 7, thr, d_c.ptr, cap, d_n.ptr)
+    c = abi.read_candidates(d_c, d_n, cap)
+    lab, nc = abi.dbscan2d(c, 3, 4)
+    peaks, sizes = abi.cluster_peaks(c, lab, nc)
+    return y, c, peaks
+
+
+def test_cfg0_pipeline_matches_fp64_oracle():
+    """Single beam, C=256, N=65536, M=256 (DM 0..25.5 step 0.1), N(0,1) noise
+    and three width-8 pulses of +0.5 per channel at DM 5.0, 12.3, 20.7."""
+    s = abi.Survey.make(418.0, -0.078125, 256, 51.2e-6, 0.0, 0.1, 256, 65536)
+    t, S = abi.delay_table(s)
+    pulses = [(5.0, 9000, 8, 0.5), (12.3, 30000, 8, 0.5), (20.7, 52000, 8, 0.5)]
+    x = O.generate(2024, 256, s.fch1, s.foff, s.tsamp, 0, 65536 + S, pulses=pulses)
+    _say("[cfg0 pipeline] GPU search")
+    y, c, peaks = _gpu_search(s, x)
+    _say("[cfg0 pipeline] FP64 oracle search")
+    y64 = O.dedisperse_f64(x, t, 65536)
+    od, ot, ok, osnr, on = O.detect(y64, 16384, 7, 6.0)
+    assert _cands_tuple(c) == list(zip(od.tolist(), ot.tolist(), ok.tolist()))
+    olab, onc = O.dbscan2d(od, ot, 3, 4)
+    assert len(peaks) == onc
+    for k in range(onc):
+        m = olab == k
+        i = np.flatnonzero(m)[np.argmax(osnr[m])]
+        assert (peaks[k]["dm_idx"], peaks[k]["sample"], peaks[k]["width_log2"]) == (od[i], ot[i], ok[i])
+    # every injected pulse is the peak of a cluster at its DM, start and width
+    for dm, t0, w, _ in pulses:
+        hit = [p for p in peaks if abs(p["dm_idx"] - round(dm / 0.1)) <= 1 and abs(p["sample"] - t0) <= 2]
+        assert hit and hit[0]["width_log2"] == 3 and hit[0]["snr"] > 15, (dm, hit)
+
+
+def test_detect_global_reference_semantics():
+    s, t, x, y = _dedispersed(13, 64, 48, 20000, pulses=[(2.0, 3000, 4, 1.0)])
+    d_y = abi.DeviceBuffer.from_numpy(y)
+    cap = 1 << 16
+    d_c, d_n = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+    mean, rms = abi.detect_global(d_y.ptr, 48, 20000, 4.0, d_c.ptr, cap, d_n.ptr)
+    om, orms = O.global_stats(y)
+    assert abs(mean - om) <= 1e-12 * max(1, abs(om)) and abs(rms - orms) <= 1e-12 * orms
+    c = abi.read_candidates(d_c, d_n, cap)
+    od, ot, osnr, on = O.threshold_ref(y, om, orms, 4.0)
+    assert list(zip(c["dm_idx"], c["sample"])) == list(zip(od, ot))
+
+
+def test_median_and_detrend():
+    rng = np.random.default_rng(14)
+    x = rng.standard_normal((7, 3001)).astype(np.float32)
+    d_x, d_o = abi.DeviceBuffer.from_numpy(x), abi.DeviceBuffer(x.nbytes)
+    abi.median_filter5(d_x.ptr, 7, 3001, d_o.ptr)
+    assert np.array_equal(d_o.download(np.float32, x.shape), O.median_filter5(x))
+    L = 4096
+    z = (0.002 * np.arange(3 * L) + 5 + rng.standard_normal(3 * L)).astype(np.float32)
+    d_z = abi.DeviceBuffer.from_numpy(z)
+    abi.detrend_normalise(d_z.ptr, 3, L)
+    assert np.allclose(d_z.download(np.float32, z.shape), O.detrend_normalise(z, L), atol=2e-6)
+
+
+def test_device_generator_matches_oracle():
+    C, n, t0 = 48, 10003, 777
+    s = abi.Survey.make(1500.0, -0.29296875 * 1024 / C, C, 64e-6, 0.0, 1.0, 4, 1024)
+    g, o = O.channel_bandpass(9, C)
+    pulses = [abi.Pulse(300.0, 2000, 12, 1.25), abi.Pulse(55.5, 9000, 1, 3.0)]
+    d_raw = abi.DeviceBuffer(C * n * 4)
+    abi.generate(9, s, t0, n, d_raw.ptr, n, g, o, pulses)
+    got = d_raw.download(np.float32, (C, n))
+    ref = O.generate(9, C, s.fch1, s.foff, s.tsamp, t0, n, g, o,
+                     pulses=[(p.dm, p.start, p.width, p.amp) for p in pulses])
+    err = np.abs(got - ref).max()
+    _say(f"[generator] max |gpu - oracle| = {err:.3e}")
+    assert err < 1e-5
