@@ -1,0 +1,390 @@
+"""Benchmark of the brute-force dedispersion hot path on one MI355X.
+
+Headline workload (one step = one pass of the dedisperser over all beams):
+32 beams of uneven sizes (nchans in {512, 1024}, nsamp uniform in
+2^17..2^20, ndms uniform in 1024..4096, dmstep uniform in 0.1..0.5,
+fch1 1500 MHz over a 300 MHz band, tsamp 64 us), Gaussian noise with
+per-channel bandpass gain U(0.5, 2) and offset U(0, 10) plus 20 injected
+pulses per beam, bandpass-normalised to FP16 storage, dedispersed by one
+persistent launch with FP32 accumulation.  The same beams are also run
+through the plain path (FP32 storage, one launch per beam) for the
+batched/FP16 vs plain/FP32 ratio, and a single full beam (1024 channels,
+2^20 samples, 4096 DMs at step 0.2) through the plain FP32 path.
+
+All inputs are generated on the device and are resident in HBM before any
+timed region; timing uses HIP events on the stream the kernels run on.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mdsm_amd import abi  # noqa: E402
+
+HBM_PEAK = 8.0e12        # MI355X HBM3E, bytes/s
+VALU_ADD_PEAK = 78.6e12  # FP32 vector adds/s with packed FP32 (157.3 TFLOP/s FMA)
+FCH1, BAND, TSAMP = 1500.0, 300.0, 64e-6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_beams(seed=2026, nbeams=32):
+    rng = np.random.default_rng(seed)
+    beams = []
+    for b in range(nbeams):
+        C = int(rng.choice([512, 1024]))
+        N = int(rng.integers(1 << 17, (1 << 20) + 1))
+        M = int(rng.integers(1024, 4097))
+        step = float(rng.uniform(0.1, 0.5))
+        s = abi.Survey.make(FCH1, -BAND / C, C, TSAMP, 0.0, step, M, N)
+        dmmax = step * (M - 1)
+        pulses = []
+        for _ in range(20):
+            dm = float(rng.uniform(10.0, min(800.0, 0.95 * dmmax)))
+            w = int(round(np.exp(rng.uniform(0, np.log(64)))))
+            snr = float(rng.uniform(8, 30))
+            t0 = int(rng.integers(0, N - 64))
+            pulses.append(abi.Pulse(dm, t0, w, snr / np.sqrt(C * w)))
+        beams.append(dict(survey=s, pulses=pulses, seed=seed * 1000 + b))
+    return beams
+
+
+def single_beam():
+    s = abi.Survey.make(FCH1, -BAND / 1024, 1024, TSAMP, 0.0, 0.2, 4096, 1 << 20)
+    rng = np.random.default_rng(7)
+    pulses = []
+    for _ in range(20):
+        w = int(round(np.exp(rng.uniform(0, np.log(64)))))
+        pulses.append(abi.Pulse(float(rng.uniform(10, 800)), int(rng.integers(0, (1 << 20) - 64)), w,
+                                float(rng.uniform(8, 30)) / np.sqrt(1024 * w)))
+    return dict(survey=s, pulses=pulses, seed=77)
+
+
+def bytes_per_beam(s, maxshift, in_bytes):
+    """Algorithmic HBM bytes: input read once, output written once, delay table."""
+    C, N, M = s.nchans, s.nsamp, s.ndms
+    return C * (N + maxshift) * in_bytes + M * N * 4 + C * M * 4
+
+
+class Stage:
+    """Generates + normalises beams into device buffers of a storage dtype."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.raw = None
+        self.raw_bytes = 0
+        self.mean = abi.DeviceBuffer(1024 * 8)
+        self.std = abi.DeviceBuffer(1024 * 8)
+
+    def fill(self, beam, dst_ptr, ld, dtype):
+        s = beam["survey"]
+        need = s.nchans * ld * 4
+        if need > self.raw_bytes:
+            if self.raw is not None:
+                self.raw.free()
+            self.raw = abi.DeviceBuffer(need)
+            self.raw_bytes = need
+        rng = np.random.default_rng(beam["seed"])
+        gain = rng.uniform(0.5, 2.0, s.nchans).astype(np.float32)
+        off = rng.uniform(0.0, 10.0, s.nchans).astype(np.float32)
+        abi.generate(beam["seed"], s, 0, ld, self.raw.ptr, ld, gain, off, beam["pulses"], self.stream)
+        abi.channel_stats(self.raw.ptr, ld, s.nchans, ld, self.mean.ptr, self.std.ptr, self.stream)
+        abi.normalise(self.raw.ptr, ld, s.nchans, ld, self.mean.ptr, self.std.ptr, dst_ptr, dtype, ld,
+                      self.stream)
+
+    def free(self):
+        if self.raw is not None:
+            self.raw.free()
+            self.raw = None
+            self.raw_bytes = 0
+
+
+def time_steps(fn, steps, warmup, stream):
+    for _ in range(warmup):
+        fn()
+    abi.synchronize()
+    t = abi.Timer(stream)
+    t.start()
+    for _ in range(steps):
+        fn()
+    ms = t.stop()
+    abi.synchronize()
+    return ms / steps
+
+
+def spot_check(beam, plan_like_table, d_in, ld, dtype, d_out, rows):
+    """Bit-exact check of selected full-length DM rows against the oracle."""
+    import oracle as O
+    s = beam["survey"]
+    np_dt = {abi.F32: np.float32, abi.F16: np.uint16, abi.U8: np.uint8}[dtype]
+    x = d_in.download(np_dt, (s.nchans, ld)) if hasattr(d_in, "download") else None
+    t = plan_like_table[:, rows]
+    if dtype == abi.F32:
+        ref = O.dedisperse_f32seq(x, np.ascontiguousarray(t), s.nsamp)
+    elif dtype == abi.F16:
+        ref = O.dedisperse_f16(x, np.ascontiguousarray(t), s.nsamp)
+    else:
+        ref = O.dedisperse_u8(x, np.ascontiguousarray(t), s.nsamp)
+    ok = True
+    for i, d in enumerate(rows):
+        got = d_out.download(np.float32, (s.nsamp,), offset=int(d) * s.nsamp * 4)
+        ok &= bool(np.array_equal(got, ref[i]))
+    return ok
+
+
+def detect_beam(beam, d_out, stream, L=16384, thr=6.0, cap=1 << 21):
+    s = beam["survey"]
+    M, N = s.ndms, s.nsamp
+    nb = (N + L - 1) // L
+    dm, ds = abi.DeviceBuffer(M * nb * 8), abi.DeviceBuffer(M * nb * 8)
+    dc, dn = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+    abi.block_stats(d_out.ptr, M, N, L, dm.ptr, ds.ptr, stream)
+    abi.detect(d_out.ptr, M, N, L, dm.ptr, ds.ptr, 7, thr, dc.ptr, cap, dn.ptr, stream)
+    abi.synchronize()
+    c = abi.read_candidates(dc, dn, cap)
+    lab, nc = abi.dbscan2d(c, 3, 4)
+    peaks, sizes = abi.cluster_peaks(c, lab, nc)
+    # recall: injected pulse matched by a cluster peak within 2% DM and one width in time
+    found = 0
+    for p in beam["pulses"]:
+        di = p.dm / s.dmstep
+        ok = [q for q in peaks if abs(q["dm_idx"] - di) <= max(2.0, 0.02 * di)
+              and abs(int(q["sample"]) - p.start) <= max(2 * p.width, 4)]
+        found += bool(ok)
+    return len(c), nc, found
+
+
+def cpu_baseline(target_s=12.0):
+    """The oracle's multithreaded FP32 channel-order brute force (the
+    reference's arithmetic, OpenMP over DM x time tiles, SIMD over time) on a
+    bounded sample: 1024 channels x 256 DM trials (step 0.2) x N samples,
+    N sized for about target_s seconds."""
+    import oracle as O
+    s = abi.Survey.make(FCH1, -BAND / 1024, 1024, TSAMP, 0.0, 0.2, 256, 1 << 20)
+    table, S = abi.delay_table(s)
+    nmax = 1 << 17
+    x = np.random.default_rng(1).standard_normal((1024, nmax + S), dtype=np.float32)
+    n0 = 8192
+    t0 = time.perf_counter()
+    O.dedisperse_f32seq(np.ascontiguousarray(x[:, :n0 + S]), table, n0)
+    el = time.perf_counter() - t0
+    N = int(min(nmax, max(n0, n0 * target_s / max(el, 1e-3))))
+    N = max(1024, (N // 1024) * 1024)
+    xs = np.ascontiguousarray(x[:, :N + S])
+    t0 = time.perf_counter()
+    O.dedisperse_f32seq(xs, table, N)
+    el = time.perf_counter() - t0
+    ref = None
+    exe = os.path.join(ROOT, "oracle", "_ref", "cpu_dedisp")
+    if os.path.exists(exe):
+        # the reference's own single-threaded brute force (src/prototypes/cpu_dedisp.c)
+        try:
+            nref = 16384
+            r = subprocess.run([exe, "-nchans", "1024", "-nsamp", str(nref), "-tdms", "64"],
+                               capture_output=True, text=True, timeout=120)
+            ms = float([ln for ln in r.stdout.splitlines() if ln.startswith("Time:")][0].split()[1])
+            ref = {"value": 64 * nref / (ms / 1e3), "unit": "DM*samples/s", "cores": 1,
+                   "kind": "reference", "sample": f"src/prototypes/cpu_dedisp.c, 1024 chans x 64 DMs x {nref} samples, time-major"}
+        except Exception as e:  # report, never fail the bench on the side measurement
+            ref = {"error": str(e)}
+    cpu = {"value": 256 * N / el, "unit": "DM*samples/s", "cores": O.num_threads(), "kind": "port",
+           "sample": f"1024 chans x 256 DMs (step 0.2) x {N} samples, FP32 channel order, "
+                     f"{el:.1f} s on {platform.processor() or platform.machine()}"}
+    return cpu, ref
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--beams", type=int, default=32)
+    ap.add_argument("--no-single", action="store_true", help="skip the single full-beam FP32 case")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-beam FP32 pass")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="only the batched FP16 launches")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the path would shard by beam; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.profile:
+        args.no_single = args.no_plain = args.no_cpu = True
+
+    stream_p = abi.C.c_void_p()
+    abi.check(abi.lib().mdsm_stream_create(abi.C.byref(stream_p)))
+    stream = stream_p.value
+    result = {}
+
+    # ---------------------------------------------------------- single beam, plain FP32
+    if not args.no_single:
+        sb = single_beam()
+        s = sb["survey"]
+        plan = abi.Plan(s, abi.F32)
+        ld = plan.min_ld
+        log(f"[single] C=1024 N=2^20 M=4096 maxshift={plan.maxshift}")
+        d_in = abi.DeviceBuffer(s.nchans * ld * 4)
+        d_out = abi.DeviceBuffer(s.ndms * s.nsamp * 4)
+        st = Stage(stream)
+        st.fill(sb, d_in.ptr, ld, abi.F32)
+        st.free()
+        abi.synchronize()
+        v, _ = plan.autotune(d_in.ptr, ld, d_out.ptr, stream)
+        ms = time_steps(lambda: plan.dedisperse(d_in.ptr, ld, d_out.ptr, stream), args.steps,
+                        args.warmup, stream)
+        work = s.ndms * s.nsamp
+        B = bytes_per_beam(s, plan.maxshift, 4)
+        t, _ = abi.delay_table(s)
+        ok = spot_check(sb, t, d_in, ld, abi.F32, d_out, [0, 1, 2047, 4095])
+        ncand, ncl, found = detect_beam(sb, d_out, stream)
+        log(f"[single] variant {v}: {ms:.2f} ms, {work / ms / 1e6:.3e} DM*samples/s, "
+            f"spot check {'ok' if ok else 'FAILED'}, {ncand} cands, {ncl} clusters, {found}/20 pulses")
+        result["single_fp32"] = {"ms": ms, "dm_samples_per_s": work / ms * 1e3, "variant": v,
+                                 "hbm_frac": B / (ms / 1e3) / HBM_PEAK,
+                                 "valu_frac": s.nchans * work / (ms / 1e3) / VALU_ADD_PEAK,
+                                 "spot_check_bitexact": ok, "candidates": ncand,
+                                 "clusters": ncl, "pulses_recovered": f"{found}/20"}
+        del d_in, d_out, plan
+
+    # ------------------------------------------------------------ batched beams
+    beams = make_beams(nbeams=args.beams)
+    surveys = [b["survey"] for b in beams]
+    outs = [abi.DeviceBuffer(s.ndms * s.nsamp * 4) for s in surveys]
+    total_work = sum(s.ndms * s.nsamp for s in surveys)
+    total_adds = sum(s.ndms * s.nsamp * s.nchans for s in surveys)
+    log(f"[batch] {len(beams)} beams, sum M*N = {total_work:.3e}, adds = {total_adds:.3e}")
+
+    def run_batch(dtype, tag):
+        batch = abi.Batch(surveys, dtype)
+        isz = {abi.F32: 4, abi.F16: 2, abi.U8: 1}[dtype]
+        st = Stage(stream)
+        ins = []
+        for b, beam in enumerate(beams):
+            ld = batch.min_ld[b]
+            d = abi.DeviceBuffer(beam["survey"].nchans * ld * isz)
+            st.fill(beam, d.ptr, ld, dtype)
+            ins.append((d, ld))
+        st.free()
+        abi.synchronize()
+        # autotune the tile shape on the largest beam truncated to 2^17 samples
+        big = max(range(len(beams)), key=lambda i: surveys[i].nchans * surveys[i].ndms)
+        sb = surveys[big]
+        probe = abi.Survey.make(sb.fch1, sb.foff, sb.nchans, sb.tsamp, sb.lowdm, sb.dmstep,
+                                sb.ndms, 1 << 17)
+        pp = abi.Plan(probe, dtype)
+        v, _ = pp.autotune(ins[big][0].ptr, ins[big][1], outs[big].ptr, stream)
+        batch.set_variant(v)
+        batch.bind([d.ptr for d, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
+        ms = time_steps(lambda: batch.launch(stream), args.steps, args.warmup, stream)
+        B = sum(bytes_per_beam(s, batch.maxshift[i], isz) for i, s in enumerate(surveys))
+        log(f"[batch {tag}] variant {v}: {ms:.1f} ms/step, {total_work / ms * 1e3:.3e} DM*samples/s, "
+            f"{B / (ms / 1e3) / 1e9:.1f} GB/s algorithmic")
+        return batch, ins, ms, B, v
+
+    batch16, ins16, ms16, B16, v16 = run_batch(abi.F16, "fp16")
+    checks = {}
+    t0tab, _ = abi.delay_table(surveys[0])
+    checks["beam0"] = spot_check(beams[0], t0tab, ins16[0][0], ins16[0][1], abi.F16, outs[0],
+                                 [0, surveys[0].ndms // 2, surveys[0].ndms - 1])
+    tl, _ = abi.delay_table(surveys[-1])
+    checks["beam_last"] = spot_check(beams[-1], tl, ins16[-1][0], ins16[-1][1], abi.F16, outs[-1],
+                                     [1, surveys[-1].ndms - 2])
+    log(f"[batch fp16] spot checks {checks}")
+    det = {"candidates": 0, "clusters": 0, "pulses_found": 0, "pulses": 0}
+    if not args.profile:
+        tm = abi.Timer(stream)
+        tm.start()
+        for b, beam in enumerate(beams):
+            n, nc, found = detect_beam(beam, outs[b], stream)
+            det["candidates"] += n
+            det["clusters"] += nc
+            det["pulses_found"] += found
+            det["pulses"] += len(beam["pulses"])
+        det["ms_incl_host_clustering"] = tm.stop()
+        log(f"[detect] {det}")
+    for d, _ in ins16:
+        d.free()
+    del batch16
+
+    plain = None
+    if not args.no_plain:
+        plans = [abi.Plan(s, abi.F32) for s in surveys]
+        st = Stage(stream)
+        ins32 = []
+        for b, beam in enumerate(beams):
+            ld = plans[b].min_ld
+            d = abi.DeviceBuffer(surveys[b].nchans * ld * 4)
+            st.fill(beam, d.ptr, ld, abi.F32)
+            ins32.append((d, ld))
+        st.free()
+        abi.synchronize()
+        for p in plans:
+            p.set_variant(v16 if v16 else 0)
+
+        def plain_step():
+            for b, p in enumerate(plans):
+                p.dedisperse(ins32[b][0].ptr, ins32[b][1], outs[b].ptr, stream)
+        ms32 = time_steps(plain_step, args.steps, args.warmup, stream)
+        B32 = sum(bytes_per_beam(s, plans[i].maxshift, 4) for i, s in enumerate(surveys))
+        plain = {"ms": ms32, "dm_samples_per_s": total_work / ms32 * 1e3,
+                 "hbm_frac": B32 / (ms32 / 1e3) / HBM_PEAK, "launches_per_step": len(plans)}
+        log(f"[plain fp32] {ms32:.1f} ms/step, {total_work / ms32 * 1e3:.3e} DM*samples/s")
+        for d, _ in ins32:
+            d.free()
+
+    cpu, cpu_ref = (None, None) if args.no_cpu else cpu_baseline()
+    if cpu:
+        log(f"[cpu] {cpu}")
+
+    value = total_work / ms16 * 1e3
+    achieved = B16 / (ms16 / 1e3)
+    line = {
+        "metric": "dedispersed DM x samples per second (batched, FP16 storage)",
+        "value": value,
+        "unit": "DM*samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms16,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "storage": "f16",
+        "data": "synthetic (seeded Philox noise, bandpass, 20 injected pulses per beam), generated on the device",
+        "config": {"workload": f"{len(beams)} uneven beams, nchans 512/1024, nsamp 2^17..2^20, "
+                               "ndms 1024..4096, dmstep 0.1..0.5, one persistent launch",
+                   "beams": len(beams), "sum_dm_samples": total_work, "sum_adds": total_adds,
+                   "tile_variant": v16},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None},
+        "valu": {"adds_per_s": total_adds / (ms16 / 1e3), "peak_packed_fp32": VALU_ADD_PEAK,
+                 "frac": total_adds / (ms16 / 1e3) / VALU_ADD_PEAK},
+        "plain_fp32_per_beam": plain,
+        "batched_fp16_vs_plain_fp32": (value / plain["dm_samples_per_s"]) if plain else None,
+        "single_beam_fp32": result.get("single_fp32"),
+        "spot_checks_bitexact": checks,
+        "detection": det,
+        "cpu_baseline": cpu,
+        "cpu_reference_1thread": cpu_ref,
+    }
+    line["roofline"]["achieved"] = achieved / 1e9
+    line["roofline"]["peak"] = HBM_PEAK / 1e9
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
