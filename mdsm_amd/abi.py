@@ -20,6 +20,7 @@ F32, F16, U8 = 0, 1, 2
 DELAY_NORTHSTAR, DELAY_REFERENCE = 0, 1
 _ITEMSIZE = {F32: 4, F16: 2, U8: 1}
 _NP = {F32: np.float32, F16: np.uint16, U8: np.uint8}
+ALIGN = {F32: 1, F16: 2, U8: 4}  # required multiple of the input row length
 
 CAND_DTYPE = np.dtype([("dm_idx", "<i4"), ("sample", "<i4"), ("width_log2", "<i4"), ("snr", "<f4")])
 
@@ -277,10 +278,16 @@ def dedisperse_host(s: Survey, x: np.ndarray, dtype: int = F32, variant: int = 0
     if variant:
         plan.set_variant(variant)
     x = np.ascontiguousarray(x, dtype=_NP[dtype])
-    assert x.shape[0] == s.nchans and x.shape[1] >= plan.min_ld
+    assert x.shape[0] == s.nchans and x.shape[1] >= s.nsamp + plan.maxshift
+    ld = max(x.shape[1], plan.min_ld)
+    ld = -(-ld // ALIGN[dtype]) * ALIGN[dtype]
+    if ld != x.shape[1]:
+        xp = np.zeros((s.nchans, ld), x.dtype)
+        xp[:, :x.shape[1]] = x
+        x = xp
     din = DeviceBuffer.from_numpy(x)
     dout = DeviceBuffer(s.ndms * s.nsamp * 4)
-    plan.dedisperse(din.ptr, x.shape[1], dout.ptr)
+    plan.dedisperse(din.ptr, ld, dout.ptr)
     synchronize()
     return dout.download(np.float32, (s.ndms, s.nsamp))
 

@@ -243,11 +243,17 @@ struct mdsm_plan_s {
   std::vector<uint32_t> row_words;  // per variant (index 1..)
 };
 
-static int default_variant(int) { return 2; }
-
-static int plan_variant(const mdsm_plan_s *p) {
-  return p->variant ? p->variant : default_variant(p->dtype);
+// Untuned default: the preferred tile shape if it can stage this survey's
+// delay spread, else the first one that can.
+static int default_variant(const mdsm_plan_s *p) {
+  const int pref = 3;
+  if (p->row_words[pref]) return pref;
+  for (int v = 1; v < (int)p->row_words.size(); ++v)
+    if (p->row_words[v]) return v;
+  return pref;
 }
+
+static int plan_variant(const mdsm_plan_s *p) { return p->variant ? p->variant : default_variant(p); }
 
 extern "C" int mdsm_plan_create(const mdsm_survey_t *s, int in_dtype, mdsm_plan *plan) {
   MDSM_REQUIRE(plan, MDSM_ERR_ARG, "null plan");
@@ -267,7 +273,8 @@ extern "C" int mdsm_plan_create(const mdsm_survey_t *s, int in_dtype, mdsm_plan 
   p->row_words.assign(nv + 1, 0);
   for (int v = 1; v <= nv; ++v) {
     const TileShape sh = variant_shape(in_dtype, v);
-    p->row_words[v] = lds_row_words(in_dtype, sh, tile_spread(p->h_table.data(), s->nchans, s->ndms, sh.td()));
+    p->row_words[v] =
+        lds_row_words(in_dtype, v, tile_spread(p->h_table.data(), s->nchans, s->ndms, sh.td()));
   }
   const size_t bytes = p->h_table.size() * sizeof(int32_t);
   hipError_t e = hipMalloc(&p->d_table, bytes);
@@ -297,7 +304,10 @@ extern "C" int mdsm_plan_destroy(mdsm_plan plan) {
 extern "C" int mdsm_plan_info(mdsm_plan plan, uint32_t *maxshift, uint64_t *min_ld) {
   MDSM_REQUIRE(plan, MDSM_ERR_ARG, "null plan");
   if (maxshift) *maxshift = plan->maxshift;
-  if (min_ld) *min_ld = (uint64_t)plan->s.nsamp + plan->maxshift;
+  if (min_ld) {
+    const uint64_t a = input_alignment(plan->dtype);
+    *min_ld = ((uint64_t)plan->s.nsamp + plan->maxshift + a - 1) / a * a;
+  }
   return MDSM_OK;
 }
 
@@ -333,6 +343,9 @@ extern "C" int mdsm_dedisperse(mdsm_plan plan, const void *d_in, uint64_t ld, fl
   MDSM_REQUIRE(ld >= (uint64_t)plan->s.nsamp + plan->maxshift, MDSM_ERR_ARG,
                "input row length %llu < nsamp + maxshift = %llu", (unsigned long long)ld,
                (unsigned long long)plan->s.nsamp + plan->maxshift);
+  MDSM_REQUIRE(ld % input_alignment(plan->dtype) == 0 && ((uintptr_t)d_in & 3u) == 0, MDSM_ERR_ARG,
+               "input rows must start on 4-byte boundaries (row length a multiple of %u)",
+               input_alignment(plan->dtype));
   const int v = plan_variant(plan);
   return launch_dedisperse(plan->dtype, v, d_in, ld, plan->d_table, plan->s.nchans, plan->s.nsamp,
                            plan->s.ndms, plan->row_words[v], d_out, as_stream(stream));

@@ -36,7 +36,20 @@ struct mdsm_batch_s {
   std::vector<BeamDesc> h_desc;
 };
 
-static int batch_variant(const mdsm_batch_s *b) { return b->variant ? b->variant : 2; }
+// Untuned default: the preferred tile shape when it fits every beam, else the
+// first shape that does.
+static int batch_variant(const mdsm_batch_s *b) {
+  if (b->variant) return b->variant;
+  auto fits = [&](int v) {
+    for (const auto &rw : b->row_words)
+      if (!rw[v]) return false;
+    return true;
+  };
+  if (fits(3)) return 3;
+  for (int v = 1; v <= num_variants(b->dtype); ++v)
+    if (fits(v)) return v;
+  return 3;
+}
 
 static void free_batch(mdsm_batch_s *b) {
   for (auto *p : b->d_tables)
@@ -70,7 +83,7 @@ extern "C" int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, in
     for (int v = 1; v <= nv; ++v) {
       const TileShape sh = variant_shape(in_dtype, v);
       b->row_words[i][v] = lds_row_words(
-          in_dtype, sh, tile_spread(b->h_tables[i].data(), beams[i].nchans, beams[i].ndms, sh.td()));
+          in_dtype, v, tile_spread(b->h_tables[i].data(), beams[i].nchans, beams[i].ndms, sh.td()));
     }
     const size_t bytes = b->h_tables[i].size() * sizeof(int32_t);
     if (hipMalloc(&b->d_tables[i], bytes) != hipSuccess ||
@@ -107,7 +120,10 @@ extern "C" int mdsm_batch_info(mdsm_batch batch, uint32_t beam, uint32_t *maxshi
                                uint64_t *min_ld) {
   MDSM_REQUIRE(batch && beam < batch->beams.size(), MDSM_ERR_ARG, "bad beam index");
   if (maxshift) *maxshift = batch->maxshift[beam];
-  if (min_ld) *min_ld = (uint64_t)batch->beams[beam].nsamp + batch->maxshift[beam];
+  if (min_ld) {
+    const uint64_t a = input_alignment(batch->dtype);
+    *min_ld = ((uint64_t)batch->beams[beam].nsamp + batch->maxshift[beam] + a - 1) / a * a;
+  }
   return MDSM_OK;
 }
 
@@ -135,10 +151,15 @@ extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const 
     MDSM_REQUIRE(d_in[i] && d_out[i], MDSM_ERR_ARG, "null buffer for beam %u", i);
     MDSM_REQUIRE(ld[i] >= (uint64_t)s.nsamp + batch->maxshift[i], MDSM_ERR_ARG,
                  "beam %u: input row length below nsamp + maxshift", i);
+    MDSM_REQUIRE(ld[i] % input_alignment(batch->dtype) == 0 && ((uintptr_t)d_in[i] & 3u) == 0,
+                 MDSM_ERR_ARG, "beam %u: input rows must start on 4-byte boundaries", i);
+    MDSM_REQUIRE(batch->row_words[i][v] > 0, MDSM_ERR_ARG,
+                 "beam %u: delay spread too large for tile variant %d", i, v);
     BeamDesc &d = batch->h_desc[k];
-    d.in = d_in[i];
-    d.out = d_out[i];
-    d.table = batch->d_tables[i];
+    const intptr_t base = reinterpret_cast<intptr_t>(batch->d_desc);
+    d.in_off = reinterpret_cast<intptr_t>(d_in[i]) - base;
+    d.out_off = reinterpret_cast<intptr_t>(d_out[i]) - base;
+    d.table_off = reinterpret_cast<intptr_t>(batch->d_tables[i]) - base;
     d.ld = ld[i];
     d.nchans = s.nchans;
     d.nsamp = s.nsamp;

@@ -1,160 +1,434 @@
 // dedisperse.hip -- brute-force dedispersion kernels for gfx950.
 //
-// out[d][t] = sum_{c=0}^{C-1} in[c][t + D[c][d]]   (FP32, channel order)
+// out[d][t] = sum_{c=0}^{C-1} in[c][t + D[c][d]]   (FP32 accumulation, channel order)
 //
-// Design (MI355X): a workgroup of 4 wave64s owns a tile of TD = 4*MW DM
-// trials x TT = 64*K samples.  Channels are processed in blocks of CB: the
-// block's input windows [t0 + D[c][d0], t0 + TT + D[c][d_last]) are staged
-// from HBM into LDS with coalesced loads, then every wave adds, for each of
-// its MW trials, K samples per lane read from LDS at lane + 64*j + (D[c][d] -
-// D[c][d0]).  The delay offsets are wave-uniform and come from the integer
-// delay table through scalar loads, so one VALU address add serves K LDS
-// reads.  Each staged sample is reused by TD trials (HBM/L2 re-reads divided
-// by TD); the kernel is bound by LDS read bandwidth (one 4-byte LDS read per
-// add), not by HBM.  No MFMA: the operation is shift-and-add, not a
-// contraction.  Accumulation order is channel 0..C-1 for every output, which
-// makes the result bit-identical to a sequential FP32 sum.
+// Design (MI355X).  A workgroup of 4 wave64s owns a tile of TD = 4*MW DM
+// trials x TT time samples; every wave owns MW consecutive trials and every
+// lane a few samples spaced 64 (FP32) or 128 (packed pairs) apart.  Channels
+// are processed in blocks: each block's input windows [t0 + D[c][d0],
+// t0 + TT + D[c][d_last]) are staged into one of two LDS buffers while the
+// waves add the previous block, so HBM/L2 latency overlaps the adds.  Each
+// staged sample is reused by TD trials.  The per-trial offsets inside a
+// window are wave-uniform; a block's offsets sit one per lane in a VGPR and
+// are moved to an SGPR with v_readlane, so one trial costs one address add
+// for K LDS reads (issued in pairs as ds_read2st64).  The adds are bound by
+// LDS read bandwidth (~128 B/clk/CU), not by HBM, so the storage formats
+// trade LDS bytes per add:
+//   F32  4 B per add: one float per LDS word, staged by direct global->LDS
+//        loads (global_load_lds_dword) that use no VGPRs.
+//   F16  2 B per add: LDS words hold pairs of consecutive samples in two
+//        copies (even and odd alignment), so any wave-uniform offset reads
+//        an aligned word; v_fma_mix_f32 (half operand selected by op_sel,
+//        times 1.0, plus the FP32 accumulator) adds one half with one IEEE
+//        round-to-nearest-even.  (v_dot2c_f32_f16 would also take one
+//        instruction but rounds exact ties differently from IEEE.)
+//   U8   2 B per add: bytes widened to pairs of u16 in the same two-copy
+//        layout, summed exactly with v_pk_add_u16 (two adds per
+//        instruction) and folded into u32 every <= 257 channels.
+// Results are bit-identical to a sequential FP32 sum in channel order (F32,
+// F16 storage) or to the exact integer sum (U8).  MFMA is not used: the
+// operation is shift-and-add, not a contraction.
 //
 // Reference counterpart: shared_dedispersion
-// (src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:44-95) with its
+// (src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:44-95) and its
 // launcher shared_brute_force (multibeam_dedispersion_thread.cu:204-221),
 // which stages one channel per two barriers for a 128 x 32 tile.
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.h"
 
 namespace mdsm {
 namespace {
 
-// Element loaders: input storage -> FP32 value that enters the sum.  U8 codes
-// are accumulated as integers held exactly in FP32 (sum < 2^24 for any
-// nchans <= 65793) and rescaled once at the end.
-struct LoadF32 {
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void glb_void;
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef unsigned short ushort2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (GFX9 encoding).
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// LDS byte offsets of the wave's trials inside the windows of one channel
+// block, relative to the block buffer plus the lane's own word.  Lane l of
+// register n holds the offset of (r, i) = divmod(64 n + l, MW), i.e. channel
+// cb+r and trial dw+i (both clamped into range), from rel = D[c][d] - D[c][d0]:
+//   PACK 0 (one sample per word):  word = r*rw + rel
+//   PACK 1 (pairs, window starts at element D[c][d0] of an even word
+//           origin): q = rel + (D[c][d0] & 1), word = r*rw + q/2 + (q&1)*rw/2
+//   PACK 2 (pairs, origin on a 4-sample boundary): as 1 with D[c][d0] & 3
+// (copy P is the first half of a row, copy Q the second).  Computed once per
+// block, so a trial costs one v_readlane and one address add.
+template <int MW, int CB, int PACK>
+struct RelOffsets {
+  static constexpr int N = (CB * MW + 63) / 64;
+  int32_t v[N];
+  __device__ __forceinline__ void load(const int32_t *__restrict__ table, uint32_t ndms,
+                                       uint32_t nchans, uint32_t cb, uint32_t d0, uint32_t dw,
+                                       uint32_t lane, uint32_t rw) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const uint32_t idx = 64u * n + lane;
+      const uint32_t r = idx / MW;
+      const uint32_t c = min(cb + r, nchans - 1u);
+      const uint32_t d = min(dw + idx % MW, ndms - 1u);
+      const int32_t *row = table + (uint64_t)c * ndms;
+      const int32_t base = row[d0];
+      const uint32_t rel = (uint32_t)(row[d] - base);
+      uint32_t w;
+      if (PACK == 0) {
+        w = r * rw + rel;
+      } else {
+        const uint32_t q = rel + ((uint32_t)base & (PACK == 1 ? 1u : 3u));
+        w = r * rw + (q >> 1) + (q & 1u) * (rw >> 1);
+      }
+      v[n] = (int32_t)(4u * w);
+    }
+  }
+  __device__ __forceinline__ int32_t get(int r, int i) const {
+    const int idx = r * MW + i;
+    return __builtin_amdgcn_readlane(v[idx >> 6], idx & 63);
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ const T *at_bytes(const T *p, int32_t bytes) {
+  return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + bytes);
+}
+
+// ======================================================================= F32
+template <int MW, int K, int CB_>
+struct F32Tile {
+  static constexpr int TT = 64 * K;
+  static constexpr int TD = 4 * MW;
+  static constexpr int CB = CB_;
+  static constexpr int ALIGN = 1;  // ld multiple, in elements
   using T = float;
-  static __device__ __forceinline__ float cvt(T v) { return v; }
-};
-struct LoadF16 {
-  using T = __half;
-  static __device__ __forceinline__ float cvt(T v) { return __half2float(v); }
-};
-struct LoadU8 {
-  using T = uint8_t;
-  static __device__ __forceinline__ float cvt(T v) { return (float)v; }
-};
 
-template <typename L>
-__device__ __forceinline__ float finish(float acc, uint32_t nchans);
-template <>
-__device__ __forceinline__ float finish<LoadF32>(float acc, uint32_t) { return acc; }
-template <>
-__device__ __forceinline__ float finish<LoadF16>(float acc, uint32_t) { return acc; }
-template <>
-__device__ __forceinline__ float finish<LoadU8>(float acc, uint32_t nchans) {
-  return (float)((double)acc * (8.0 / 255.0) + (double)nchans * -4.0);
-}
+  static uint32_t row_words(uint32_t spread) { return ((uint32_t)TT + spread + 63u) & ~63u; }
+  static uint32_t lds_words_per_row(uint32_t rw) { return 2u * rw; }
+  static bool fits(uint32_t) { return true; }
 
-__device__ __forceinline__ uint32_t uniform(uint32_t v) {
-  return __builtin_amdgcn_readfirstlane(v);
-}
+  __device__ __forceinline__ static void run(const float *__restrict__ in, uint64_t ld,
+                             const int32_t *__restrict__ table, uint32_t nchans, uint32_t nsamp,
+                             uint32_t ndms, uint32_t rw, uint32_t cbr, float *__restrict__ out,
+                             uint32_t tt, uint32_t dt, float *lds) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = uniform(threadIdx.x >> 6);
+    const uint32_t t0 = tt * TT, d0 = dt * TD;
+    const uint32_t dlast = min(d0 + TD, ndms) - 1u;
+    const uint32_t dw = d0 + wave * MW;
+    const uint32_t bufw = cbr * rw;  // words per LDS buffer (offsets keep the LDS address space)
 
-// One tile.  lds: CB rows of row_words floats.
-template <typename L, int MW, int K, int CB>
-__device__ __forceinline__ void dedisp_tile(const typename L::T *__restrict__ in, uint64_t ld,
-                                            const int32_t *__restrict__ table, uint32_t nchans,
-                                            uint32_t nsamp, uint32_t ndms, uint32_t row_words,
-                                            uint32_t cbr, float *__restrict__ out, uint32_t tt,
-                                            uint32_t dt, float *lds) {
-  constexpr int TT = 64 * K;
-  constexpr int TD = 4 * MW;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t lane = tid & 63u;
-  const uint32_t wave = uniform(tid >> 6);
-  const uint32_t t0 = tt * TT;
-  const uint32_t d0 = dt * TD;
-  const uint32_t dlast = min(d0 + TD, ndms) - 1u;
-  const uint32_t dw = d0 + wave * MW;  // this wave's first trial (may exceed ndms)
-
-  float acc[MW][K];
-#pragma unroll
-  for (int i = 0; i < MW; ++i)
-#pragma unroll
-    for (int j = 0; j < K; ++j) acc[i][j] = 0.0f;
-
-  // cbr (<= CB) channels per stage: CB unless the delay spread limits LDS
-  for (uint32_t cb = 0; cb < nchans; cb += cbr) {
-    const uint32_t nc = min(cbr, nchans - cb);
-    // ---- stage the block's windows into LDS (coalesced, zero beyond the row)
-    for (uint32_t r = 0; r < nc; ++r) {
-      const uint32_t c = cb + r;
-      const int32_t *trow = table + (uint64_t)c * ndms;
-      const uint32_t base = (uint32_t)trow[d0];
-      const uint32_t len = TT + (uint32_t)trow[dlast] - base;
-      const uint64_t start = (uint64_t)t0 + base;
-      const typename L::T *src = in + (uint64_t)c * ld;
-      float *dst = lds + r * row_words;
-      for (uint32_t i = tid; i < len; i += 256u) {
-        const uint64_t g = start + i;
-        dst[i] = g < ld ? L::cvt(src[g]) : 0.0f;
+    // wave w copies 64-word chunks w, w+4, ... of each row from global
+    // memory straight into LDS; lanes past the end of the row stay idle
+    // (their words only feed outputs beyond nsamp)
+    auto stage = [&](uint32_t cb, float *b) {
+      const uint32_t nc = min(cbr, nchans - cb);
+      for (uint32_t r = 0; r < nc; ++r) {
+        const int32_t *trow = table + (uint64_t)(cb + r) * ndms;
+        const int32_t base = trow[d0];
+        const uint32_t len = (uint32_t)TT + (uint32_t)(trow[dlast] - base);
+        const uint64_t g0 = (uint64_t)t0 + (uint32_t)base;
+        const float *src = in + (uint64_t)(cb + r) * ld + g0;
+        const uint32_t nq = (len + 63u) >> 6;
+        for (uint32_t q = wave; q < nq; q += 4u) {
+          if (g0 + 64u * q + lane < ld)
+            __builtin_amdgcn_global_load_lds((glb_void *)(src + 64u * q + lane),
+                                             (lds_void *)(b + r * rw + 64u * q), 4, 0, 0);
+        }
       }
-    }
+    };
+
+    float acc[MW][K];
+#pragma unroll
+    for (int i = 0; i < MW; ++i)
+#pragma unroll
+      for (int j = 0; j < K; ++j) acc[i][j] = 0.0f;
+
+    RelOffsets<MW, CB, 0> rel, rel_next;
+    stage(0, lds);
+    rel.load(table, ndms, nchans, 0, d0, dw, lane, rw);
+    wait_vmem();
     __syncthreads();
-    // ---- accumulate: channel order is preserved for every output
-    for (uint32_t r = 0; r < nc; ++r) {
-      const int32_t *trow = table + (uint64_t)(cb + r) * ndms;
-      const int32_t base = trow[d0];
-      const float *row = lds + r * row_words + lane;
-#pragma unroll
-      for (int i = 0; i < MW; ++i) {
-        const uint32_t d = min(dw + (uint32_t)i, ndms - 1u);
-        const int32_t rel = trow[d] - base;
-        const float *p = row + rel;
-#pragma unroll
-        for (int j = 0; j < K; ++j) acc[i][j] += p[64 * j];
+    uint32_t k = 0;
+    for (uint32_t cb = 0; cb < nchans; cb += cbr, ++k) {
+      const float *cur = lds + (k & 1u) * bufw + lane;
+      const bool more = cb + cbr < nchans;
+      if (more) {
+        stage(cb + cbr, lds + ((k + 1u) & 1u) * bufw);
+        rel_next.load(table, ndms, nchans, cb + cbr, d0, dw, lane, rw);
       }
+      const uint32_t nc = min(cbr, nchans - cb);
+#pragma unroll
+      for (int r = 0; r < CB; ++r) {
+        if ((uint32_t)r < nc) {
+#pragma unroll
+          for (int i = 0; i < MW; ++i) {
+            const float *p = at_bytes(cur, rel.get(r, i));
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[i][j] += p[64 * j];
+          }
+        }
+      }
+      wait_vmem();
+      __syncthreads();
+      if (more) rel = rel_next;
     }
-    __syncthreads();
-  }
 
 #pragma unroll
-  for (int i = 0; i < MW; ++i) {
-    const uint32_t d = dw + (uint32_t)i;
-    if (d < ndms) {
-      float *o = out + (uint64_t)d * nsamp;
+    for (int i = 0; i < MW; ++i) {
+      const uint32_t d = dw + (uint32_t)i;
+      if (d < ndms) {
+        float *o = out + (uint64_t)d * nsamp;
 #pragma unroll
-      for (int j = 0; j < K; ++j) {
-        const uint32_t t = t0 + lane + 64u * j;
-        if (t < nsamp) o[t] = finish<L>(acc[i][j], nchans);
+        for (int j = 0; j < K; ++j) {
+          const uint32_t t = t0 + lane + 64u * j;
+          if (t < nsamp) o[t] = acc[i][j];
+        }
       }
     }
   }
-}
+};
 
-template <typename L, int MW, int K, int CB>
-__global__ __launch_bounds__(256) void dedisp_kernel(const typename L::T *__restrict__ in,
+// ============================================================ packed (F16, U8)
+// LDS row of a channel: copy P (words 0..wp-1) then copy Q (words wp..2wp-1).
+// With f the even word origin of the row window (elements 2f, 2f+1, ... of
+// the global row), P[w] = (e[2(f+w)], e[2(f+w)+1]) and Q[w] = (e[2(f+w)+1],
+// e[2(f+w)+2]) as two 16-bit halves.  For the window start a = g0 + rel of a
+// trial (wave-uniform), the lane's pair (a + 2 lane + 128 j, +1) is word
+// (a>>1) - f + lane + 64 j of copy (a & 1).
+template <int MW, int K2, int CB_>
+struct PackedGeom {
+  static constexpr int TT = 128 * K2;
+  static constexpr int TD = 4 * MW;
+  static constexpr int CB = CB_;
+  static uint32_t copy_words(uint32_t spread) {
+    return ((uint32_t)(64 * K2) + (spread + 4u) / 2u + 2u + 63u) & ~63u;
+  }
+  static uint32_t row_words(uint32_t spread) { return 2u * copy_words(spread); }
+};
+
+// Packed tiles (F16 and U8 storage, rows 4-byte aligned).  Per channel block
+// the raw dwords of each window are copied into LDS by direct global->LDS
+// loads while the previous block is added; after the barrier one LDS->LDS
+// pass builds the pair copies (F16: P is the copy itself and Q its 16-bit
+// funnel shift; U8: bytes widened to u16 pairs with v_perm into P and Q).
+template <int MODE, int MW, int K2, int CB_>
+struct PackedTile : PackedGeom<MW, K2, CB_> {
+  using G = PackedGeom<MW, K2, CB_>;
+  static constexpr int TT = G::TT, TD = G::TD, CB = CB_;
+  static constexpr bool IS_U8 = MODE == MDSM_U8;
+  using T = std::conditional_t<IS_U8, uint8_t, __half>;
+  static bool fits(uint32_t) { return true; }
+  // raw staging dwords per row (U8): wp/2 + 2 source dwords, padded to 64
+  static __host__ __device__ uint32_t raw_words(uint32_t rw) {
+    return IS_U8 ? ((rw / 4u + 2u + 63u) & ~63u) : 0u;
+  }
+  static uint32_t row_words(uint32_t spread) { return G::row_words(spread); }
+  static uint32_t lds_words_per_row(uint32_t rw) { return 2u * rw + raw_words(rw); }
+
+  __device__ __forceinline__ static void run(const T *__restrict__ in, uint64_t ld,
+                                             const int32_t *__restrict__ table, uint32_t nchans,
+                                             uint32_t nsamp, uint32_t ndms, uint32_t rw,
+                                             uint32_t cbr, float *__restrict__ out, uint32_t tt,
+                                             uint32_t dt, float *ldsf) {
+    uint32_t *lds = reinterpret_cast<uint32_t *>(ldsf);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = uniform(tid >> 6);
+    const uint32_t t0 = tt * TT, d0 = dt * TD;
+    const uint32_t dw = d0 + wave * MW;
+    const uint32_t wp = rw / 2u;
+    const uint32_t rawn = raw_words(rw);
+    const uint64_t ldw = IS_U8 ? ld / 4u : ld / 2u;  // source dwords per row
+    const uint32_t *src32 = reinterpret_cast<const uint32_t *>(in);
+    const uint32_t bufw = cbr * rw;  // words per LDS buffer (offsets keep the LDS address space)
+    uint32_t *raw = lds + 2u * cbr * rw;
+
+    auto g0_of = [&](uint32_t c) { return t0 + (uint32_t)table[(uint64_t)c * ndms + d0]; };
+    // copy the window dwords of rows [cb, cb+nc) into P of buffer b (F16) or
+    // into the raw area (U8); wave w takes 64-dword chunks w, w+4, ...
+    auto stage = [&](uint32_t cb, uint32_t *b) {
+      const uint32_t nc = min(cbr, nchans - cb);
+      for (uint32_t r = 0; r < nc; ++r) {
+        const uint32_t g0 = g0_of(cb + r);
+        const uint64_t w0 = IS_U8 ? (g0 >> 2) : (g0 >> 1);
+        const uint32_t *src = src32 + (uint64_t)(cb + r) * ldw + w0;
+        uint32_t *dst = IS_U8 ? raw + r * rawn : b + r * rw;
+        const uint32_t n = IS_U8 ? rawn : wp;
+        for (uint32_t q = wave; 64u * q < n; q += 4u) {
+          if (w0 + 64u * q + lane < ldw)
+            __builtin_amdgcn_global_load_lds((glb_void *)(src + 64u * q + lane),
+                                             (lds_void *)(dst + 64u * q), 4, 0, 0);
+        }
+      }
+    };
+    // build the pair copies of rows [cb, cb+nc) in buffer b
+    auto build = [&](uint32_t cb, uint32_t *b) {
+      const uint32_t nc = min(cbr, nchans - cb);
+      for (uint32_t r = 0; r < nc; ++r) {
+        uint32_t *P = b + r * rw;
+        uint32_t *Q = P + wp;
+        if (IS_U8) {
+          const uint32_t *R = raw + r * rawn;
+          for (uint32_t n = tid; 2u * n < wp; n += 256u) {
+            const uint32_t x = R[n], y = R[n + 1];
+            P[2 * n] = __builtin_amdgcn_perm(y, x, 0x0C010C00u);
+            P[2 * n + 1] = __builtin_amdgcn_perm(y, x, 0x0C030C02u);
+            Q[2 * n] = __builtin_amdgcn_perm(y, x, 0x0C020C01u);
+            Q[2 * n + 1] = __builtin_amdgcn_perm(y, x, 0x0C040C03u);
+          }
+        } else {
+          for (uint32_t w = tid; w + 1u < wp; w += 256u)
+            Q[w] = __builtin_amdgcn_alignbit(P[w + 1], P[w], 16);
+        }
+      }
+    };
+
+    // F16: FP32 accumulators, U8: u16 pair accumulators folded into u32
+    float facc[IS_U8 ? 1 : MW][IS_U8 ? 1 : K2][2];
+    ushort2_t uacc[IS_U8 ? MW : 1][IS_U8 ? K2 : 1];
+    uint32_t usum[IS_U8 ? MW : 1][IS_U8 ? K2 : 1][2];
+#pragma unroll
+    for (int i = 0; i < MW; ++i)
+#pragma unroll
+      for (int j = 0; j < K2; ++j) {
+        if (IS_U8) {
+          uacc[i][j] = ushort2_t{0, 0};
+          usum[i][j][0] = usum[i][j][1] = 0u;
+        } else {
+          facc[i][j][0] = facc[i][j][1] = 0.0f;
+        }
+      }
+    auto fold = [&]() {
+      if (IS_U8) {
+#pragma unroll
+        for (int i = 0; i < MW; ++i)
+#pragma unroll
+          for (int j = 0; j < K2; ++j) {
+            usum[i][j][0] += uacc[i][j].x;
+            usum[i][j][1] += uacc[i][j].y;
+            uacc[i][j] = ushort2_t{0, 0};
+          }
+      }
+    };
+    // 1.0 from an opaque scalar move: fma(fpext(h), one, acc) then selects
+    // v_fma_mix_f32 instead of being folded into a convert plus an add
+    uint32_t one_bits;
+    asm("s_mov_b32 %0, 0x3f800000" : "=s"(one_bits));
+    const float one = __builtin_bit_cast(float, one_bits);
+
+    RelOffsets<MW, CB, IS_U8 ? 2 : 1> rel, rel_next;
+    stage(0, lds);
+    rel.load(table, ndms, nchans, 0, d0, dw, lane, rw);
+    wait_vmem();
+    __syncthreads();
+    build(0, lds);
+    __syncthreads();
+    uint32_t k = 0, since_fold = 0;
+    for (uint32_t cb = 0; cb < nchans; cb += cbr, ++k) {
+      const uint32_t *cur = lds + (k & 1u) * bufw + lane;
+      const bool more = cb + cbr < nchans;
+      if (more) {
+        stage(cb + cbr, lds + ((k + 1u) & 1u) * bufw);
+        rel_next.load(table, ndms, nchans, cb + cbr, d0, dw, lane, rw);
+      }
+      const uint32_t nc = min(cbr, nchans - cb);
+      if (IS_U8) {
+        if (since_fold + nc > 257u) {  // 257 * 255 = 65535 still fits in 16 bits
+          fold();
+          since_fold = 0;
+        }
+        since_fold += nc;
+      }
+#pragma unroll
+      for (int r = 0; r < CB; ++r) {
+        if ((uint32_t)r < nc) {
+#pragma unroll
+          for (int i = 0; i < MW; ++i) {
+            const uint32_t *p = at_bytes(cur, rel.get(r, i));
+#pragma unroll
+            for (int j = 0; j < K2; ++j) {
+              const uint32_t word = p[64 * j];
+              if (IS_U8) {
+                uacc[i][j] += __builtin_bit_cast(ushort2_t, word);
+              } else {
+                const half2_t h = __builtin_bit_cast(half2_t, word);
+                facc[i][j][0] = __builtin_fmaf((float)h.x, one, facc[i][j][0]);
+                facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
+              }
+            }
+          }
+        }
+      }
+      wait_vmem();
+      __syncthreads();
+      if (more) {
+        build(cb + cbr, lds + ((k + 1u) & 1u) * bufw);
+        __syncthreads();
+        rel = rel_next;
+      }
+    }
+    fold();
+
+    const double base = (double)nchans * -4.0;
+#pragma unroll
+    for (int i = 0; i < MW; ++i) {
+      const uint32_t d = dw + (uint32_t)i;
+      if (d < ndms) {
+        float *o = out + (uint64_t)d * nsamp;
+#pragma unroll
+        for (int j = 0; j < K2; ++j) {
+          const uint32_t t = t0 + 2u * lane + 128u * j;
+          float v0, v1;
+          if (IS_U8) {
+            v0 = (float)((double)usum[i][j][0] * (8.0 / 255.0) + base);
+            v1 = (float)((double)usum[i][j][1] * (8.0 / 255.0) + base);
+          } else {
+            v0 = facc[i][j][0];
+            v1 = facc[i][j][1];
+          }
+          if (t < nsamp) o[t] = v0;
+          if (t + 1u < nsamp) o[t + 1] = v1;
+        }
+      }
+    }
+  }
+};
+
+template <int MW, int K2, int CB>
+using F16Tile = PackedTile<MDSM_F16, MW, K2, CB>;
+template <int MW, int K2, int CB>
+using U8Tile = PackedTile<MDSM_U8, MW, K2, CB>;
+
+// ================================================================= kernels
+template <typename Tile>
+__global__ __launch_bounds__(256) void dedisp_kernel(const typename Tile::T *__restrict__ in,
                                                      uint64_t ld,
                                                      const int32_t *__restrict__ table,
                                                      uint32_t nchans, uint32_t nsamp,
-                                                     uint32_t ndms, uint32_t row_words,
-                                                     uint32_t cbr, uint32_t n_dtiles,
-                                                     float *__restrict__ out) {
+                                                     uint32_t ndms, uint32_t rw, uint32_t cbr,
+                                                     uint32_t n_dtiles, float *__restrict__ out) {
   extern __shared__ float lds[];
-  // DM tiles vary fastest so that the workgroups resident at one time share
-  // a narrow time range of the input (reuse in L2 and the 256 MB MALL).
+  // DM tiles vary fastest: the workgroups resident at one time then share a
+  // narrow time range of the input (reuse in L2 and the 256 MB MALL).
   const uint32_t b = blockIdx.x;
-  dedisp_tile<L, MW, K, CB>(in, ld, table, nchans, nsamp, ndms, row_words, cbr, out, b / n_dtiles,
-                            b % n_dtiles, lds);
+  Tile::run(in, ld, table, nchans, nsamp, ndms, rw, cbr, out, b / n_dtiles, b % n_dtiles, lds);
 }
 
 // Persistent multi-beam kernel: workgroups pull items from a queue until it
-// is exhausted.  Items are ordered by the host (largest beams first); item i
-// belongs to the beam with the largest item_begin <= i.
-template <typename L, int MW, int K, int CB>
+// is exhausted (every wave of every workgroup reaches the exit).  Items are
+// ordered by the host (largest beams first); item i belongs to the beam with
+// the largest item_begin <= i.
+template <typename Tile>
 __global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__ beams,
                                                     uint32_t nbeams, uint32_t total_items,
-                                                    uint32_t cbr,
-                                                    uint32_t *__restrict__ counter) {
+                                                    uint32_t cbr, uint32_t *__restrict__ counter) {
   extern __shared__ float lds[];
   __shared__ uint32_t item_s;
   while (true) {
@@ -162,7 +436,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__
     __syncthreads();
     const uint32_t item = item_s;
     __syncthreads();
-    if (item >= total_items) break;  // every wave of every workgroup reaches this
+    if (item >= total_items) break;
     uint32_t lo = 0, hi = nbeams - 1u;
     while (lo < hi) {
       const uint32_t mid = (lo + hi + 1u) >> 1;
@@ -171,152 +445,146 @@ __global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__
     }
     const BeamDesc &bd = beams[lo];
     const uint32_t local = item - bd.item_begin;
-    dedisp_tile<L, MW, K, CB>(reinterpret_cast<const typename L::T *>(bd.in), bd.ld, bd.table,
-                              bd.nchans, bd.nsamp, bd.ndms, bd.row_words, cbr, bd.out,
-                              local / bd.n_dtiles, local % bd.n_dtiles, lds);
+    char *base = reinterpret_cast<char *>(const_cast<BeamDesc *>(beams));
+    Tile::run(reinterpret_cast<const typename Tile::T *>(base + bd.in_off), bd.ld,
+              reinterpret_cast<const int32_t *>(base + bd.table_off), bd.nchans, bd.nsamp,
+              bd.ndms, bd.row_words, cbr, reinterpret_cast<float *>(base + bd.out_off),
+              local / bd.n_dtiles, local % bd.n_dtiles, lds);
+    __syncthreads();
   }
 }
 
 // ------------------------------------------------------------ variant table
-// Channels staged per barrier: the variant's CB, reduced when CB rows of the
-// delay-spread window would not fit in 64 KB of LDS (at most 160 KB per CU).
-uint32_t stage_channels(int cb, uint32_t row_words) {
-  const uint64_t budget = 64 * 1024 / sizeof(float);
-  uint64_t n = (uint64_t)cb;
-  while (n > 1 && n * row_words > budget) --n;
-  if (n * row_words * sizeof(float) > 160 * 1024) return 0;
+// Variant v: F32 tile (MW, K, CB) and packed tiles (MW, K2, CB) with the same
+// TD = 4*MW and similar time extent.
+template <int V>
+struct VariantDef;
+template <> struct VariantDef<1> { using F32 = F32Tile<8, 4, 8>;   using F16 = F16Tile<8, 2, 8>;   using U8 = U8Tile<8, 2, 8>; };
+template <> struct VariantDef<2> { using F32 = F32Tile<16, 4, 4>;  using F16 = F16Tile<16, 2, 4>;  using U8 = U8Tile<16, 2, 4>; };
+template <> struct VariantDef<3> { using F32 = F32Tile<8, 8, 8>;   using F16 = F16Tile<8, 4, 8>;   using U8 = U8Tile<8, 4, 8>; };
+template <> struct VariantDef<4> { using F32 = F32Tile<16, 8, 4>;  using F16 = F16Tile<16, 4, 4>;  using U8 = U8Tile<16, 4, 4>; };
+template <> struct VariantDef<5> { using F32 = F32Tile<32, 2, 2>;  using F16 = F16Tile<32, 1, 2>;  using U8 = U8Tile<32, 1, 2>; };
+constexpr int kNumVariants = 5;
+
+template <typename V, typename Fn>
+int pick_dtype(int dtype, Fn &&fn) {
+  switch (dtype) {
+    case MDSM_F32: return fn((typename V::F32 *)nullptr);
+    case MDSM_F16: return fn((typename V::F16 *)nullptr);
+    case MDSM_U8: return fn((typename V::U8 *)nullptr);
+  }
+  set_error("unknown input dtype %d", dtype);
+  return MDSM_ERR_ARG;
+}
+
+template <typename Fn>
+int with_tile(int dtype, int variant, Fn &&fn) {
+  switch (variant) {
+    case 1: return pick_dtype<VariantDef<1>>(dtype, fn);
+    case 2: return pick_dtype<VariantDef<2>>(dtype, fn);
+    case 3: return pick_dtype<VariantDef<3>>(dtype, fn);
+    case 4: return pick_dtype<VariantDef<4>>(dtype, fn);
+    case 5: return pick_dtype<VariantDef<5>>(dtype, fn);
+  }
+  set_error("unknown dedispersion variant %d", variant);
+  return MDSM_ERR_ARG;
+}
+
+// Channels staged per block: the tile's CB, reduced while the block's LDS
+// (two buffers, plus raw staging for U8) exceeds 64 KB (160 KB per CU).
+template <typename Tile>
+uint32_t stage_channels(uint32_t rw) {
+  const uint64_t per = Tile::lds_words_per_row(rw) * sizeof(float);
+  uint64_t n = (uint64_t)Tile::CB;
+  while (n > 1 && n * per > 64 * 1024) --n;
+  if (n * per > 160 * 1024) return 0;
   return (uint32_t)n;
-}
-
-// Variant 1..N (0 = use the plan's tuned choice).
-const TileShape kShapes[] = {
-    {8, 4, 8},   // 1: TD 32,  TT 256
-    {16, 4, 8},  // 2: TD 64,  TT 256
-    {8, 8, 8},   // 3: TD 32,  TT 512
-    {16, 8, 4},  // 4: TD 64,  TT 512
-    {32, 2, 8},  // 5: TD 128, TT 128
-};
-constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
-
-template <typename L>
-int launch_one(int variant, const void *in, uint64_t ld, const int32_t *table, uint32_t nchans,
-               uint32_t nsamp, uint32_t ndms, uint32_t row_words, float *out, hipStream_t s) {
-  const TileShape sh = kShapes[variant - 1];
-  const uint32_t n_ttiles = (nsamp + sh.tt() - 1) / sh.tt();
-  const uint32_t n_dtiles = (ndms + sh.td() - 1) / sh.td();
-  const uint64_t blocks = (uint64_t)n_ttiles * n_dtiles;
-  MDSM_REQUIRE(blocks < (1ull << 31), MDSM_ERR_ARG, "dedispersion grid too large");
-  const uint32_t cbr = stage_channels(sh.cb, row_words);
-  MDSM_REQUIRE(cbr >= 1, MDSM_ERR_ARG,
-               "delay spread too large for LDS staging (%u words per channel)", row_words);
-  const size_t lds = (size_t)cbr * row_words * sizeof(float);
-  const auto *src = reinterpret_cast<const typename L::T *>(in);
-#define MDSM_CASE(V, MW, K, CB)                                                                \
-  case V:                                                                                      \
-    hipLaunchKernelGGL((dedisp_kernel<L, MW, K, CB>), dim3((uint32_t)blocks), dim3(256), lds, s, \
-                       src, ld, table, nchans, nsamp, ndms, row_words, cbr, n_dtiles, out);   \
-    break;
-  switch (variant) {
-    MDSM_CASE(1, 8, 4, 8)
-    MDSM_CASE(2, 16, 4, 8)
-    MDSM_CASE(3, 8, 8, 8)
-    MDSM_CASE(4, 16, 8, 4)
-    MDSM_CASE(5, 32, 2, 8)
-    default:
-      set_error("unknown dedispersion variant %d", variant);
-      return MDSM_ERR_ARG;
-  }
-#undef MDSM_CASE
-  MDSM_HIP(hipGetLastError());
-  return MDSM_OK;
-}
-
-template <typename L, int MW, int K, int CB>
-int launch_batch_shape(const BeamDesc *beams, uint32_t nbeams, uint32_t total, uint32_t *counter,
-                       uint32_t cbr, size_t lds, int grid, hipStream_t s) {
-  if (grid <= 0) {
-    // persistent: as many workgroups as can be resident at once
-    int per_cu = 0, dev = 0, cus = 0;
-    MDSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<L, MW, K, CB>,
-                                                          256, lds));
-    MDSM_HIP(hipGetDevice(&dev));
-    MDSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    grid = std::max(1, per_cu) * std::max(1, cus);
-  }
-  hipLaunchKernelGGL((batch_kernel<L, MW, K, CB>), dim3(grid), dim3(256), lds, s, beams, nbeams,
-                     total, cbr, counter);
-  MDSM_HIP(hipGetLastError());
-  return MDSM_OK;
-}
-
-template <typename L>
-int launch_batch_one(int variant, const BeamDesc *beams, uint32_t nbeams, uint32_t total,
-                     uint32_t *counter, uint32_t cbr, size_t lds, int grid, hipStream_t s) {
-#define MDSM_CASE(V, MW, K, CB)                                                                \
-  case V:                                                                                      \
-    return launch_batch_shape<L, MW, K, CB>(beams, nbeams, total, counter, cbr, lds, grid, s);
-  switch (variant) {
-    MDSM_CASE(1, 8, 4, 8)
-    MDSM_CASE(2, 16, 4, 8)
-    MDSM_CASE(3, 8, 8, 8)
-    MDSM_CASE(4, 16, 8, 4)
-    MDSM_CASE(5, 32, 2, 8)
-    default:
-      set_error("unknown dedispersion variant %d", variant);
-      return MDSM_ERR_ARG;
-  }
-#undef MDSM_CASE
 }
 
 }  // namespace
 
-int num_variants(int) { return kNumShapes; }
+int num_variants(int) { return kNumVariants; }
 
-TileShape variant_shape(int, int variant) { return kShapes[variant - 1]; }
+TileShape variant_shape(int dtype, int variant) {
+  TileShape sh{0, 0, 0};
+  with_tile(dtype, variant, [&](auto *tp) {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    sh.mw = Tile::TD / 4;
+    sh.k = Tile::TT / 64;
+    sh.cb = Tile::CB;
+    return MDSM_OK;
+  });
+  return sh;
+}
 
-uint32_t lds_row_words(int, const TileShape &sh, uint32_t spread) {
-  const uint32_t w = (uint32_t)sh.tt() + spread;
-  return (w + 3u) & ~3u;
+uint32_t lds_row_words(int dtype, int variant, uint32_t spread) {
+  uint32_t rw = 0;
+  with_tile(dtype, variant, [&](auto *tp) {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    rw = Tile::fits(spread) ? Tile::row_words(spread) : 0u;
+    return MDSM_OK;
+  });
+  return rw;
+}
+
+uint32_t input_alignment(int dtype) {
+  return dtype == MDSM_F32 ? 1u : (dtype == MDSM_F16 ? 2u : 4u);
 }
 
 int launch_dedisperse(int dtype, int variant, const void *in, uint64_t ld, const int32_t *table,
-                      uint32_t nchans, uint32_t nsamp, uint32_t ndms, uint32_t row_words,
-                      float *out, hipStream_t s) {
-  MDSM_REQUIRE(variant >= 1 && variant <= kNumShapes, MDSM_ERR_ARG, "bad variant %d", variant);
-  switch (dtype) {
-    case MDSM_F32:
-      return launch_one<LoadF32>(variant, in, ld, table, nchans, nsamp, ndms, row_words, out, s);
-    case MDSM_F16:
-      return launch_one<LoadF16>(variant, in, ld, table, nchans, nsamp, ndms, row_words, out, s);
-    case MDSM_U8:
-      return launch_one<LoadU8>(variant, in, ld, table, nchans, nsamp, ndms, row_words, out, s);
-  }
-  set_error("unknown input dtype %d", dtype);
-  return MDSM_ERR_ARG;
+                      uint32_t nchans, uint32_t nsamp, uint32_t ndms, uint32_t rw, float *out,
+                      hipStream_t s) {
+  MDSM_REQUIRE(rw > 0, MDSM_ERR_ARG,
+               "delay spread too large for tile variant %d (choose another variant)", variant);
+  return with_tile(dtype, variant, [&](auto *tp) -> int {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    const uint32_t n_ttiles = (nsamp + Tile::TT - 1) / Tile::TT;
+    const uint32_t n_dtiles = (ndms + Tile::TD - 1) / Tile::TD;
+    const uint64_t blocks = (uint64_t)n_ttiles * n_dtiles;
+    MDSM_REQUIRE(blocks < (1ull << 31), MDSM_ERR_ARG, "dedispersion grid too large");
+    const uint32_t cbr = stage_channels<Tile>(rw);
+    MDSM_REQUIRE(cbr >= 1, MDSM_ERR_ARG, "delay spread too large for LDS staging (%u words)", rw);
+    const size_t lds = (size_t)cbr * Tile::lds_words_per_row(rw) * sizeof(float);
+    hipLaunchKernelGGL(dedisp_kernel<Tile>, dim3((uint32_t)blocks), dim3(256), lds, s,
+                       reinterpret_cast<const typename Tile::T *>(in), ld, table, nchans, nsamp,
+                       ndms, rw, cbr, n_dtiles, out);
+    MDSM_HIP(hipGetLastError());
+    return MDSM_OK;
+  });
 }
 
-size_t batch_lds_bytes(int, int variant, uint32_t max_row_words) {
-  return (size_t)stage_channels(kShapes[variant - 1].cb, max_row_words) * max_row_words *
-         sizeof(float);
+size_t batch_lds_bytes(int dtype, int variant, uint32_t max_rw) {
+  size_t bytes = 0;
+  with_tile(dtype, variant, [&](auto *tp) {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    bytes = (size_t)stage_channels<Tile>(max_rw) * Tile::lds_words_per_row(max_rw) * sizeof(float);
+    return MDSM_OK;
+  });
+  return bytes;
 }
 
 int launch_batch_dedisperse(int dtype, int variant, const BeamDesc *d_beams, uint32_t nbeams,
                             const uint32_t *, uint32_t total_items, uint32_t *d_counter,
-                            uint32_t max_row_words, int grid, hipStream_t s) {
-  MDSM_REQUIRE(variant >= 1 && variant <= kNumShapes, MDSM_ERR_ARG, "bad variant %d", variant);
-  const uint32_t cbr = stage_channels(kShapes[variant - 1].cb, max_row_words);
-  MDSM_REQUIRE(cbr >= 1, MDSM_ERR_ARG, "delay spread too large for LDS staging");
-  const size_t lds = (size_t)cbr * max_row_words * sizeof(float);
-  switch (dtype) {
-    case MDSM_F32:
-      return launch_batch_one<LoadF32>(variant, d_beams, nbeams, total_items, d_counter, cbr, lds, grid, s);
-    case MDSM_F16:
-      return launch_batch_one<LoadF16>(variant, d_beams, nbeams, total_items, d_counter, cbr, lds, grid, s);
-    case MDSM_U8:
-      return launch_batch_one<LoadU8>(variant, d_beams, nbeams, total_items, d_counter, cbr, lds, grid, s);
-  }
-  set_error("unknown input dtype %d", dtype);
-  return MDSM_ERR_ARG;
+                            uint32_t max_rw, int grid, hipStream_t s) {
+  MDSM_REQUIRE(max_rw > 0, MDSM_ERR_ARG, "delay spread too large for tile variant %d", variant);
+  return with_tile(dtype, variant, [&](auto *tp) -> int {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    const uint32_t cbr = stage_channels<Tile>(max_rw);
+    MDSM_REQUIRE(cbr >= 1, MDSM_ERR_ARG, "delay spread too large for LDS staging");
+    const size_t lds = (size_t)cbr * Tile::lds_words_per_row(max_rw) * sizeof(float);
+    if (grid <= 0) {
+      // persistent: as many workgroups as can be resident at once
+      int per_cu = 0, dev = 0, cus = 0;
+      MDSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<Tile>, 256, lds));
+      MDSM_HIP(hipGetDevice(&dev));
+      MDSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      grid = std::max(1, per_cu) * std::max(1, cus);
+    }
+    hipLaunchKernelGGL(batch_kernel<Tile>, dim3(grid), dim3(256), lds, s, d_beams, nbeams,
+                       total_items, cbr, d_counter);
+    MDSM_HIP(hipGetLastError());
+    return MDSM_OK;
+  });
 }
 
 }  // namespace mdsm

@@ -50,8 +50,11 @@ TileShape variant_shape(int dtype, int variant);
 // max_c max_{tile} D[c][min(d0+td,M)-1] - D[c][d0].
 uint32_t tile_spread(const int32_t *table, uint32_t nchans, uint32_t ndms, int td);
 
-// LDS words per staged channel row for a shape (samples + spread, padded).
-uint32_t lds_row_words(int dtype, const TileShape &sh, uint32_t spread);
+// LDS words per staged channel row of a variant for a delay spread; 0 when
+// the variant cannot stage that spread.
+uint32_t lds_row_words(int dtype, int variant, uint32_t spread);
+// Required multiple (in elements) of the input row length: 1 (F32), 2 (F16), 4 (U8).
+uint32_t input_alignment(int dtype);
 
 // Validates survey parameters and fills the host delay table.
 int build_delay_table(const mdsm_survey_t *s, std::vector<int32_t> &table, uint32_t *maxshift);
@@ -61,11 +64,14 @@ int launch_dedisperse(int dtype, int variant, const void *in, uint64_t ld, const
                       uint32_t nchans, uint32_t nsamp, uint32_t ndms, uint32_t row_words,
                       float *out, hipStream_t stream);
 
-// Per-beam descriptor of the persistent batched dedisperser.
+// Per-beam descriptor of the persistent batched dedisperser.  Buffers are
+// given as byte offsets from the descriptor array itself: pointers derived
+// from a kernel argument keep the global address space (global_load rather
+// than flat_load, which would also count against the LDS wait counter).
 struct BeamDesc {
-  const void *in;
-  float *out;
-  const int32_t *table;
+  int64_t in_off;
+  int64_t out_off;
+  int64_t table_off;
   uint64_t ld;
   uint32_t nchans, nsamp, ndms, row_words;
   uint32_t n_ttiles, n_dtiles;

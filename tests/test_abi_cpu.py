@@ -99,3 +99,53 @@ def test_dbscan_ref_matches_oracle():
         lab, k = abi.dbscan_ref(tm, dm, snr, 0.05, 1.0, 1.5, mp)
         olab, ok = O.dbscan_ref(tm, dm, snr, 0.05, 1.0, 1.5, mp)
         assert k == ok and np.array_equal(lab, olab)
+
+
+# ---------------------------------------------------------------- XML surveys
+import ctypes as _C
+_lib = abi.lib()
+_lib.mdsm_compat_parse_xml.restype = _C.c_int
+_lib.mdsm_compat_parse_xml.argtypes = [_C.c_char_p, _C.POINTER(abi.Survey), _C.c_uint32,
+                                       _C.POINTER(_C.c_uint32), _C.POINTER(_C.c_float),
+                                       _C.POINTER(_C.c_uint32), _C.POINTER(_C.c_int)]
+
+
+def _parse(name):
+    beams = (abi.Survey * 8)()
+    nb, thr, mp, dia = _C.c_uint32(), _C.c_float(), _C.c_uint32(), _C.c_int()
+    path = os.path.join(ROOT, "tests", "golden", name).encode()
+    rc = _lib.mdsm_compat_parse_xml(path, beams, 8, _C.byref(nb), _C.byref(thr), _C.byref(mp), _C.byref(dia))
+    assert rc == 0
+    return list(beams[:nb.value]), thr.value, mp.value, dia.value
+
+
+def test_xml_multibeam_dialect():
+    beams, thr, mp, dia = _parse("survey_multibeam.xml")
+    assert dia == 1 and len(beams) == 2
+    b = beams[1]
+    assert (b.fch1, b.foff, b.nchans, b.ndms, b.nsamp) == (1400.0, -0.25, 128, 300, 8192)
+    assert abs(b.tsamp - 6.4e-5) < 1e-12 and abs(b.lowdm - 2.5) < 1e-7 and abs(b.dmstep - 0.25) < 1e-7
+    assert abs(thr - 6.5) < 1e-6 and mp == 3
+
+
+def test_xml_legacy_attribute_names():
+    # the multibeam reader of the reference would read nsamp=0 and threshold=0
+    # here; both names are accepted and the dialect is reported
+    beams, thr, mp, dia = _parse("survey_legacy_names.xml")
+    assert dia & 2 and len(beams) == 1
+    b = beams[0]
+    assert (b.nchans, b.ndms, b.nsamp) == (256, 512, 16384)
+    assert abs(b.fch1 - 399.951172) < 1e-4 and abs(thr - 4.0) < 1e-6 and mp == 15
+
+
+def test_xml_errors():
+    beams = (abi.Survey * 1)()
+    nb = _C.c_uint32()
+    assert _lib.mdsm_compat_parse_xml(b"/nonexistent.xml", beams, 1, _C.byref(nb), None, None, None) != 0
+    bad = os.path.join(ROOT, "_work", "bad.xml") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp/bad.xml"
+    with open(bad, "w") as f:
+        f.write("<notobservation><dm lowDM='0'/></notobservation>")
+    assert _lib.mdsm_compat_parse_xml(bad.encode(), beams, 1, _C.byref(nb), None, None, None) != 0
+    with open(bad, "w") as f:
+        f.write("<observation><dm lowDM='0' </observation>")
+    assert _lib.mdsm_compat_parse_xml(bad.encode(), beams, 1, _C.byref(nb), None, None, None) != 0

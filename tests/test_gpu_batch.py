@@ -34,7 +34,7 @@ def test_batch_uneven_beams_bitexact(dtype):
     rng = np.random.default_rng(dtype)
     ins, outs, refs, keep = [], [], [], []
     for b, s in enumerate(beams):
-        ld = batch.min_ld[b] + 5
+        ld = batch.min_ld[b] + 8
         x = rng.standard_normal((s.nchans, ld)).astype(np.float32)
         t, _ = abi.delay_table(s)
         if dtype == abi.F32:
