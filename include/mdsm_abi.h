@@ -33,7 +33,9 @@ enum {
   MDSM_ERR_HIP = 2,      /* a HIP runtime call failed */
   MDSM_ERR_NOMEM = 3,    /* device or host allocation failed */
   MDSM_ERR_OVERFLOW = 4, /* a bounded output (candidate list) overflowed */
-  MDSM_ERR_STATE = 5     /* call out of sequence */
+  MDSM_ERR_STATE = 5,    /* call out of sequence */
+  MDSM_ERR_IO = 6,       /* a file could not be opened or read */
+  MDSM_ERR_FORMAT = 7    /* malformed SIGPROC header */
 };
 
 /* Input sample storage types. */
@@ -222,6 +224,87 @@ int mdsm_batch_launch(mdsm_batch batch, void *stream);
 int mdsm_batch_dedisperse(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                           float *const *d_out, void *stream);
 int mdsm_batch_set_variant(mdsm_batch batch, int variant);
+
+/* ------------------------------------------------------- search pipeline
+ * The per-chunk sequence of a multi-beam search: for every beam bandpass
+ * normalisation of the raw FP32 input into the storage type, one persistent
+ * batched dedispersion, then per-block statistics and boxcar detection into
+ * a per-beam candidate list.  The reference runs these stages from separate
+ * host threads (multibeam_dedispersion_thread.cu:464-839,
+ * multibeam_dedispersion_output.cpp:325-452).  mdsm_pipeline_run records the
+ * sequence into a hipGraph on its first call with use_graph set and replays
+ * the graph afterwards (counters are cleared by a kernel inside the graph).
+ * The pipeline owns the normalised inputs, outputs and candidate buffers;
+ * raw inputs are bound by the caller and must stay valid while the graph is
+ * replayed. */
+typedef struct {
+  uint32_t stats_block; /* samples per statistics block (e.g. 16384) */
+  int32_t nwidths;      /* boxcar widths 1..2^(nwidths-1), <= 7 */
+  double thresh;        /* S/N threshold */
+  uint32_t cand_cap;    /* candidates kept per beam */
+} mdsm_detect_params_t;
+typedef struct mdsm_pipeline_s *mdsm_pipeline;
+int mdsm_pipeline_create(const mdsm_survey_t *beams, uint32_t nbeams, int storage_dtype,
+                         const mdsm_detect_params_t *params, mdsm_pipeline *pipeline);
+int mdsm_pipeline_destroy(mdsm_pipeline pipeline);
+/* d_raw[b]: beam b raw input [nchans_b][ld_raw[b]] FP32 with
+ * ld_raw[b] >= nsamp_b + maxshift_b (see mdsm_batch_info). */
+int mdsm_pipeline_bind(mdsm_pipeline pipeline, const float *const *d_raw, const uint64_t *ld_raw);
+int mdsm_pipeline_set_variant(mdsm_pipeline pipeline, int variant);
+int mdsm_pipeline_run(mdsm_pipeline pipeline, void *stream, int use_graph);
+/* Sorted candidates of beam b from the last run (host copy); *n receives
+ * the total found (may exceed cap, then MDSM_ERR_OVERFLOW is returned). */
+int mdsm_pipeline_candidates(mdsm_pipeline pipeline, uint32_t beam, mdsm_cand_t *host,
+                             uint32_t cap, uint32_t *n);
+/* Device pointer to beam b's dedispersed series [ndms][nsamp]. */
+int mdsm_pipeline_output(mdsm_pipeline pipeline, uint32_t beam, float **d_out);
+
+/* ------------------------------------------------------- SIGPROC ingest
+ * Filterbank files: a keyword header (read_header,
+ * src/mdsm-multibeam/file_handler.cpp:56-190) followed by time-major
+ * spectra of nbits-wide samples (1/2/4 bits packed least-significant first,
+ * 8/16 unsigned, 32 float; read_block, file_handler.cpp:193-287).  The
+ * reference unpacks on the host and corner-turns into [beam][chan][t]
+ * float rows (readBinaryData, src/MDSM.cpp:71-104); here packed bytes cross
+ * PCIe and one kernel unpacks and transposes them into the device rows. */
+typedef struct {
+  char source_name[81], rawdatafile[81];
+  int32_t machine_id, telescope_id, data_type, nchans, nbits, nifs;
+  int32_t nbeams, ibeam, barycentric, pulsarcentric, nbins, nsamples;
+  double tstart, tsamp, fch1, foff, refdm, az_start, za_start, src_raj, src_dej, period;
+  int64_t npuls;
+  int32_t nfchannel;           /* entries of frequency_table (fchannel keywords) */
+  int32_t has_header;          /* 0: no HEADER_START, raw 8-bit data from byte 0 */
+  int32_t total_bytes;         /* the reference's FILE_HEADER.total_bytes count */
+  int64_t data_offset;         /* byte offset of the first sample */
+  double frequency_table[4096];
+} mdsm_sigproc_header_t;
+/* Parses the header of `path` (MDSM_ERR_FORMAT on a malformed or unknown
+ * keyword, where the reference exits the process).  A file without
+ * HEADER_START is raw 8-bit data (src/MDSM.cpp:53-56). */
+int mdsm_sigproc_read_header(const char *path, mdsm_sigproc_header_t *header);
+/* Unpack + corner turn on the device: d_packed holds nspectra time-major
+ * spectra of nchans_total = nbeams*nchans samples (nchans_total*nbits a
+ * multiple of 8).  Sample (t, b*nchans + c) is written as float to
+ * d_out[b*beam_stride + c*ld + col0 + t]. */
+int mdsm_unpack_corner_turn(const void *d_packed, int nbits, uint32_t nspectra, uint32_t nchans,
+                            uint32_t nbeams, float *d_out, uint64_t ld, uint64_t beam_stride,
+                            uint64_t col0, void *stream);
+/* Streaming reader: opens the file at its first sample and owns two pinned
+ * + device staging slots of max_spectra spectra.  nchans is the channel
+ * count per beam (0: the header's nchans / nbeams); the header, when given,
+ * receives the parsed header. */
+typedef struct mdsm_sigproc_reader_s *mdsm_sigproc_reader;
+int mdsm_sigproc_open(const char *path, uint32_t nchans, uint32_t nbeams, uint32_t max_spectra,
+                      mdsm_sigproc_reader *reader, mdsm_sigproc_header_t *header);
+int mdsm_sigproc_close(mdsm_sigproc_reader reader);
+/* Reads up to nspectra whole spectra (fewer at the end of the file; *got
+ * receives the count), copies the packed bytes to the device and unpacks
+ * them as mdsm_unpack_corner_turn does, on `stream`.  Returns once the file
+ * read is done and the copy and unpack are queued; the slot is reused two
+ * calls later, after its event. */
+int mdsm_sigproc_read(mdsm_sigproc_reader reader, uint32_t nspectra, float *d_out, uint64_t ld,
+                      uint64_t beam_stride, uint64_t col0, uint32_t *got, void *stream);
 
 /* ------------------------------------------------------- synthetic input
  * Seeded synthetic filterbank data generated on the device (Philox4x32-10

@@ -15,7 +15,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmdsm_amd.so")
 
-MDSM_OK, MDSM_ERR_ARG, MDSM_ERR_HIP, MDSM_ERR_NOMEM, MDSM_ERR_OVERFLOW, MDSM_ERR_STATE = range(6)
+(MDSM_OK, MDSM_ERR_ARG, MDSM_ERR_HIP, MDSM_ERR_NOMEM, MDSM_ERR_OVERFLOW, MDSM_ERR_STATE,
+ MDSM_ERR_IO, MDSM_ERR_FORMAT) = range(8)
 F32, F16, U8 = 0, 1, 2
 DELAY_NORTHSTAR, DELAY_REFERENCE = 0, 1
 _ITEMSIZE = {F32: 4, F16: 2, U8: 1}
@@ -445,3 +446,129 @@ def generate(seed, s: Survey, t0, n, d_raw, ld, gain, offset, pulses=(), stream=
         args = (None, None, None, None)
     check(_lib.mdsm_generate(seed, s.nchans, t0, n, _ptr(gain), _ptr(offset), len(pulses), *args,
                              d_raw, ld, stream))
+
+
+# --------------------------------------------------------------- pipeline
+class DetectParams(C.Structure):
+    _fields_ = [("stats_block", C.c_uint32), ("nwidths", C.c_int32), ("thresh", C.c_double),
+                ("cand_cap", C.c_uint32)]
+
+
+_sig("mdsm_pipeline_create", _P(Survey), _u32, _i, _P(DetectParams), _P(_vp))
+_sig("mdsm_pipeline_destroy", _vp)
+_sig("mdsm_pipeline_bind", _vp, _P(_vp), _P(_u64))
+_sig("mdsm_pipeline_set_variant", _vp, _i)
+_sig("mdsm_pipeline_run", _vp, _vp, _i)
+_sig("mdsm_pipeline_candidates", _vp, _u32, _vp, _u32, _P(_u32))
+_sig("mdsm_pipeline_output", _vp, _u32, _P(_vp))
+
+
+class Pipeline:
+    """normalise -> batched dedispersion -> block statistics -> detection,
+    optionally recorded into a hipGraph and replayed."""
+
+    def __init__(self, surveys, dtype=F16, stats_block=16384, nwidths=7, thresh=6.0, cand_cap=1 << 20):
+        arr = (Survey * len(surveys))(*surveys)
+        self.params = DetectParams(stats_block, nwidths, thresh, cand_cap)
+        p = _vp()
+        check(_lib.mdsm_pipeline_create(arr, len(surveys), dtype, C.byref(self.params), C.byref(p)))
+        self.handle = p.value
+        self.surveys = list(surveys)
+        self.cap = cand_cap
+
+    def bind(self, d_raw, lds):
+        n = len(self.surveys)
+        self._keep = ((_vp * n)(*d_raw), (_u64 * n)(*lds))
+        check(_lib.mdsm_pipeline_bind(self.handle, *self._keep))
+
+    def set_variant(self, v):
+        check(_lib.mdsm_pipeline_set_variant(self.handle, v))
+
+    def run(self, stream=None, graph=False):
+        check(_lib.mdsm_pipeline_run(self.handle, stream, 1 if graph else 0))
+
+    def candidates(self, beam):
+        n = _u32(0)
+        out = np.zeros(self.cap, CAND_DTYPE)
+        check(_lib.mdsm_pipeline_candidates(self.handle, beam, _ptr(out), self.cap, C.byref(n)))
+        return out[:n.value].copy()
+
+    def output_ptr(self, beam):
+        p = _vp()
+        check(_lib.mdsm_pipeline_output(self.handle, beam, C.byref(p)))
+        return p.value
+
+    def close(self):
+        if self.handle:
+            check(_lib.mdsm_pipeline_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ SIGPROC ingest
+class SigprocHeader(C.Structure):
+    _fields_ = ([("source_name", C.c_char * 81), ("rawdatafile", C.c_char * 81)] +
+                [(k, C.c_int32) for k in ("machine_id", "telescope_id", "data_type", "nchans", "nbits",
+                                          "nifs", "nbeams", "ibeam", "barycentric", "pulsarcentric",
+                                          "nbins", "nsamples")] +
+                [(k, C.c_double) for k in ("tstart", "tsamp", "fch1", "foff", "refdm", "az_start",
+                                           "za_start", "src_raj", "src_dej", "period")] +
+                [("npuls", C.c_int64), ("nfchannel", C.c_int32), ("has_header", C.c_int32),
+                 ("total_bytes", C.c_int32), ("data_offset", C.c_int64),
+                 ("frequency_table", C.c_double * 4096)])
+
+
+_sig("mdsm_sigproc_read_header", C.c_char_p, _P(SigprocHeader))
+_sig("mdsm_unpack_corner_turn", _vp, _i, _u32, _u32, _u32, _vp, _u64, _u64, _u64, _vp)
+_sig("mdsm_sigproc_open", C.c_char_p, _u32, _u32, _u32, _P(_vp), _P(SigprocHeader))
+_sig("mdsm_sigproc_close", _vp)
+_sig("mdsm_sigproc_read", _vp, _u32, _vp, _u64, _u64, _u64, _P(_u32), _vp)
+
+
+def sigproc_read_header(path: str) -> SigprocHeader:
+    h = SigprocHeader()
+    check(_lib.mdsm_sigproc_read_header(path.encode(), C.byref(h)))
+    return h
+
+
+def unpack_corner_turn(d_packed, nbits, nspectra, nchans, nbeams, d_out, ld, beam_stride=None,
+                       col0=0, stream=None):
+    beam_stride = nchans * ld if beam_stride is None else beam_stride
+    check(_lib.mdsm_unpack_corner_turn(d_packed, nbits, nspectra, nchans, nbeams, d_out, ld,
+                                       beam_stride, col0, stream))
+
+
+class SigprocReader:
+    """Streaming filterbank reader: packed spectra -> device [beam][chan][t] rows."""
+
+    def __init__(self, path: str, nchans: int = 0, nbeams: int = 1, max_spectra: int = 65536):
+        self.handle = _vp()
+        self.header = SigprocHeader()
+        check(_lib.mdsm_sigproc_open(path.encode(), nchans, nbeams, max_spectra, C.byref(self.handle),
+                                     C.byref(self.header)))
+        self.nbeams = nbeams
+        self.nchans = nchans or self.header.nchans // nbeams
+
+    def read(self, nspectra, d_out, ld, beam_stride=None, col0=0, stream=None) -> int:
+        beam_stride = self.nchans * ld if beam_stride is None else beam_stride
+        got = _u32(0)
+        check(_lib.mdsm_sigproc_read(self.handle, nspectra, d_out, ld, beam_stride, col0,
+                                     C.byref(got), stream))
+        return got.value
+
+    def close(self):
+        if self.handle:
+            check(_lib.mdsm_sigproc_close(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

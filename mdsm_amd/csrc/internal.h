@@ -102,6 +102,9 @@ int launch_threshold_global(const float *x, uint32_t nrows, uint32_t nsamp, doub
 int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t s);
 int launch_median5(const float *x, uint32_t nrows, uint32_t nsamp, float *out, hipStream_t s);
 int launch_detrend(float *x, uint64_t nseg, uint32_t L, hipStream_t s);
+int launch_unpack_corner_turn(const void *in, int nbits, uint32_t nspec, uint32_t nchans,
+                              uint32_t nbeams, float *out, uint64_t ld, uint64_t beam_stride,
+                              uint64_t col0, hipStream_t s);
 int launch_generate(uint64_t seed, uint32_t nchans, int64_t t0, uint64_t n, const float *gain,
                     const float *offset, uint32_t npulses, const int64_t *p_start,
                     const int32_t *p_width, const float *p_amp, const int64_t *p_delay, float *raw,

@@ -87,6 +87,8 @@ _sig("oracle_process_brute_legacy", _i64, _f32p, _i, _i, _f, _f, _f, _i64, _i32p
      _f32p, C.POINTER(_f), C.POINTER(_f))
 _sig("oracle_detrend_normalise", None, _f32p, _i64, _i, _f32p)
 _sig("oracle_num_threads", _i)
+_sig("oracle_sigproc_unpack", None, _u8p, _i, _i64, _f32p)
+_sig("oracle_corner_turn", None, _f32p, _i64, _i, _i, _f32p, _i64, _i64, _i64)
 
 KDM = 4148.808
 U8_SCALE = 8.0 / 255.0
@@ -337,3 +339,94 @@ def detrend_normalise(x, L):
 
 def num_threads():
     return int(_lib.oracle_num_threads())
+
+
+def sigproc_unpack(raw_bytes, nbits, n):
+    """Unpack n samples of width nbits from a byte string (read_block)."""
+    b = np.frombuffer(bytes(raw_bytes), np.uint8).copy()
+    need = (n * nbits + 7) // 8
+    if b.size < need:
+        raise ValueError("not enough bytes")
+    out = np.empty(n, np.float32)
+    _lib.oracle_sigproc_unpack(b if b.size else np.zeros(1, np.uint8), nbits, n, out)
+    return out
+
+
+def corner_turn(spectra, nchans, nbeams, ld=None, beam_stride=None, col0=0, out=None):
+    """[t][nbeams*nchans] -> out[b*beam_stride + c*ld + col0 + t] (readBinaryData)."""
+    spectra = _c(spectra, np.float32).reshape(-1, nchans * nbeams)
+    nspec = spectra.shape[0]
+    ld = nspec + col0 if ld is None else ld
+    beam_stride = nchans * ld if beam_stride is None else beam_stride
+    if out is None:
+        out = np.zeros(nbeams * beam_stride, np.float32)
+    _lib.oracle_corner_turn(spectra, nspec, nchans, nbeams, out, ld, beam_stride, col0)
+    return out
+
+
+# reference file_handler.cpp compiled from its own source (oracle/Makefile
+# target ref); present only where the MDSM sources are.
+_REF_FILE = os.path.join(_HERE, "_ref", "libref_file.so")
+
+
+def ref_file_available():
+    return os.path.exists(_REF_FILE)
+
+
+def ref_read_header(path):
+    """The reference read_header on `path`: dict or None (no HEADER_START)."""
+    lib = C.CDLL(_REF_FILE)
+    lib.ref_read_header.restype = C.c_int
+    lib.ref_read_header.argtypes = [C.c_char_p] + [C.POINTER(_d)] * 4 + [C.POINTER(_i)] * 4 + [C.c_char_p]
+    d = [_d() for _ in range(4)]
+    i = [_i() for _ in range(4)]
+    name = C.create_string_buffer(96)
+    r = lib.ref_read_header(path.encode(), *[C.byref(x) for x in d], *[C.byref(x) for x in i], name)
+    if r < 0:
+        raise OSError(path)
+    if r == 0:
+        return None
+    keys = ["fch1", "foff", "tsamp", "tstart"]
+    out = {k: v.value for k, v in zip(keys, d)}
+    out.update({k: v.value for k, v in zip(["nchans", "nbits", "nifs", "total_bytes"], i)})
+    out["source_name"] = name.value.decode()
+    return out
+
+
+REF_HEADER_FIELDS = ["machine_id", "telescope_id", "data_type", "nchans", "nbits", "nifs", "nbeams",
+                     "ibeam", "barycentric", "pulsarcentric", "nbins", "nsamples", "tstart", "tsamp",
+                     "fch1", "foff", "refdm", "az_start", "za_start", "src_raj", "src_dej", "period",
+                     "npuls", "total_bytes", "file_position"]
+
+
+def ref_header_values(path, ntable=0):
+    """All header fields the reference read_header fills (dict), or None."""
+    lib = C.CDLL(_REF_FILE)
+    lib.ref_header_values.restype = C.c_int
+    lib.ref_header_values.argtypes = [C.c_char_p, _f64p, C.c_char_p, C.c_char_p, _f64p, _i]
+    v = np.zeros(len(REF_HEADER_FIELDS), np.float64)
+    src = C.create_string_buffer(96)
+    raw = C.create_string_buffer(96)
+    table = np.zeros(max(ntable, 1), np.float64)
+    r = lib.ref_header_values(path.encode(), v, src, raw, table, ntable)
+    if r < 0:
+        raise OSError(path)
+    if r == 0:
+        return None
+    out = dict(zip(REF_HEADER_FIELDS, v.tolist()))
+    # unset strings hold uninitialised bytes in the reference
+    out["source_name"] = src.value.decode(errors="replace")
+    out["rawdatafile"] = raw.value.decode(errors="replace")
+    out["frequency_table"] = table[:ntable]
+    return out
+
+
+def ref_read_block(path, byte_offset, nbits, nread):
+    """The reference read_block: up to nread samples from byte_offset."""
+    lib = C.CDLL(_REF_FILE)
+    lib.ref_read_block.restype = C.c_long
+    lib.ref_read_block.argtypes = [C.c_char_p, C.c_long, _i, _f32p, C.c_ulong]
+    out = np.zeros(nread, np.float32)
+    got = lib.ref_read_block(path.encode(), byte_offset, nbits, out, nread)
+    return out[:got]
+

@@ -24,6 +24,8 @@
  *   DBSCAN (3-D box variant)    src/mdsm-multibeam/dbscan.cpp:194-276,390-415
  *   median of five (legacy)     src/mdsm/dedispersion_output.cpp:14-53,135-213
  *   detrend + normalise         src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:207-295
+ *   SIGPROC sample unpack       src/mdsm-multibeam/file_handler.cpp:193-287
+ *   corner turn into beams      src/MDSM.cpp:71-104
  * North-star extensions restated from their own definitions (no reference
  * code exists for them): FP64 round() delay table with 4148.808, per-block
  * FP64 mean/RMS, dyadic boxcar detection, 2-D Chebyshev DBSCAN, FP16/uint8
@@ -45,6 +47,9 @@
  *                      text (the reference needs Qt headers: unbuildable here)
  *   median5            sorted median over random and exhaustive-order inputs
  *   f16 conversion     numpy float16 (round to nearest even)
+ *   sigproc unpack     the reference's read_block compiled from its own
+ *                      source (oracle/_ref/libref_file.so) and the values
+ *                      the fixture writer packed (tests/golden)
  */
 #include <math.h>
 #include <omp.h>
@@ -773,3 +778,38 @@ void oracle_detrend_normalise(const float *x, int64_t nseg, int L, float *out) {
 }
 
 int oracle_num_threads(void) { return omp_get_max_threads(); }
+
+/* SIGPROC sample unpack (read_block, file_handler.cpp:193-287) over a
+ * contiguous stream of n samples: 1, 2 and 4-bit fields are taken from the
+ * least significant bits of each byte upward, 8 and 16-bit samples are
+ * unsigned, 32-bit samples are IEEE floats. */
+void oracle_sigproc_unpack(const uint8_t *bytes, int nbits, int64_t n, float *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (nbits < 8) {
+      int64_t bit = i * nbits;
+      unsigned v = (bytes[bit >> 3] >> (bit & 7)) & ((1u << nbits) - 1u);
+      out[i] = (float)v;
+    } else if (nbits == 8) {
+      out[i] = (float)bytes[i];
+    } else if (nbits == 16) {
+      uint16_t v;
+      memcpy(&v, bytes + 2 * i, 2);
+      out[i] = (float)v;
+    } else {
+      memcpy(out + i, bytes + 4 * i, 4);
+    }
+  }
+}
+
+/* Corner turn of time-major spectra [t][nbeams*nchans] into beam rows
+ * (readBinaryData, src/MDSM.cpp:71-104): out[b*beam_stride + c*ld + col0 + t]
+ * = spectra[t*nbeams*nchans + b*nchans + c]. */
+void oracle_corner_turn(const float *spectra, int64_t nspectra, int nchans, int nbeams, float *out,
+                        int64_t ld, int64_t beam_stride, int64_t col0) {
+  const int64_t ct = (int64_t)nchans * nbeams;
+  for (int64_t t = 0; t < nspectra; ++t)
+    for (int b = 0; b < nbeams; ++b)
+      for (int c = 0; c < nchans; ++c)
+        out[b * beam_stride + c * ld + col0 + t] = spectra[t * ct + (int64_t)b * nchans + c];
+}
+
