@@ -259,6 +259,60 @@ int mdsm_pipeline_candidates(mdsm_pipeline pipeline, uint32_t beam, mdsm_cand_t 
 /* Device pointer to beam b's dedispersed series [ndms][nsamp]. */
 int mdsm_pipeline_output(mdsm_pipeline pipeline, uint32_t beam, float **d_out);
 
+/* --------------------------------------------------- bandpass and RFI
+ * The reference's bandpass_fitting + rfi_clipping stage
+ * (multibeam_dedispersion_thread.cu:224-390; kernels
+ * multibeam_dedispersion_kernel.cu:302-450) on raw [nchans][ld] FP32 rows,
+ * over the n samples from column col0 (the reference's `shift`).  Device
+ * kernels keep the reference's reduction orders (bit-identical results);
+ * the polynomial fit runs on the host in FP64. */
+typedef struct {
+  uint32_t ncoeffs;         /* polynomial terms (reference default 8) */
+  uint32_t channel_block;   /* samples per channel block (default 1024) */
+  float channel_thresh;     /* block flagged above rz(fit) + this * rmse (default 10) */
+  float spectrum_thresh;    /* spectrum clipped outside mean +- this * std (default 6) */
+  uint32_t nmasks;          /* channel ranges replaced by their neighbours */
+  const int32_t *mask_from, *mask_to;
+  int32_t apply_channel_clip; /* 0: flags computed, not applied (the reference
+                                 clears them before channel_clipper,
+                                 multibeam_dedispersion_thread.cu:364) */
+} mdsm_rfi_params_t;
+typedef struct {
+  float mean, std, rmse;    /* of the masked bandpass; rmse against the fit */
+  float channel_thresh;     /* channel_thresh * rmse */
+  float spectrum_hi, spectrum_lo;
+} mdsm_bandpass_stats_t;
+/* d_bandpass[c] = mean of row c (bandpass_power_sum, kernel.cu:317-350). */
+int mdsm_bandpass_sum(const float *d_x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                      double *d_bandpass, void *stream);
+/* Host: masks, least-squares fit at x_i = i/nchans, statistics and
+ * thresholds (thread.cu:224-296, 344-346).  fitted: host double[nchans]. */
+int mdsm_bandpass_fit(const double *bandpass, uint32_t nchans, const mdsm_rfi_params_t *params,
+                      double *fitted, mdsm_bandpass_stats_t *stats);
+/* d_flags: uint8 [nchans][ceil(n/channel_block)], cleared by the call
+ * (channel_flagger, kernel.cu:353-398). */
+int mdsm_rfi_channel_flag(const float *d_x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                          const double *d_fitted, uint32_t channel_block, float thresh,
+                          uint8_t *d_flags, void *stream);
+/* channel_clipper, kernel.cu:402-423. */
+int mdsm_rfi_channel_clip(float *d_x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                          const double *d_fitted, uint32_t channel_block, const uint8_t *d_flags,
+                          void *stream);
+/* spectrum_clipper, kernel.cu:427-450; d_count (may be NULL) is incremented
+ * per clipped spectrum. */
+int mdsm_rfi_spectrum_clip(float *d_x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                           const double *d_fitted, float hi, float lo, uint32_t *d_count,
+                           void *stream);
+/* voltage_to_power, kernel.cu:302-312: (re, im) int16 pairs -> power. */
+int mdsm_voltage_to_power(float *d_x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                          void *stream);
+/* The whole stage: sum, host fit, flag (when d_flags is given), clip.
+ * d_fitted: device double[nchans] (receives the fitted bandpass).
+ * Synchronises the stream (the fit needs the sums on the host). */
+int mdsm_rfi_clean(float *d_x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                   const mdsm_rfi_params_t *params, double *d_fitted, uint8_t *d_flags,
+                   mdsm_bandpass_stats_t *stats, void *stream);
+
 /* ------------------------------------------------------- SIGPROC ingest
  * Filterbank files: a keyword header (read_header,
  * src/mdsm-multibeam/file_handler.cpp:56-190) followed by time-major

@@ -572,3 +572,54 @@ class SigprocReader:
         except Exception:
             pass
 
+
+# ------------------------------------------------------------ bandpass / RFI
+class RfiParams(C.Structure):
+    _fields_ = [("ncoeffs", C.c_uint32), ("channel_block", C.c_uint32), ("channel_thresh", C.c_float),
+                ("spectrum_thresh", C.c_float), ("nmasks", C.c_uint32), ("mask_from", _P(C.c_int32)),
+                ("mask_to", _P(C.c_int32)), ("apply_channel_clip", C.c_int32)]
+
+    @classmethod
+    def make(cls, ncoeffs=8, channel_block=1024, channel_thresh=10.0, spectrum_thresh=6.0, masks=(),
+             apply_channel_clip=False):
+        p = cls(ncoeffs, channel_block, channel_thresh, spectrum_thresh, len(masks), None, None,
+                1 if apply_channel_clip else 0)
+        if masks:
+            p._fr = (C.c_int32 * len(masks))(*[m[0] for m in masks])
+            p._to = (C.c_int32 * len(masks))(*[m[1] for m in masks])
+            p.mask_from = C.cast(p._fr, _P(C.c_int32))
+            p.mask_to = C.cast(p._to, _P(C.c_int32))
+        return p
+
+
+class BandpassStats(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("mean", "std", "rmse", "channel_thresh", "spectrum_hi",
+                                         "spectrum_lo")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_sig("mdsm_bandpass_sum", _vp, _u64, _u32, _u64, _u64, _vp, _vp)
+_sig("mdsm_bandpass_fit", _vp, _u32, _P(RfiParams), _vp, _P(BandpassStats))
+_sig("mdsm_rfi_channel_flag", _vp, _u64, _u32, _u64, _u64, _vp, _u32, _f, _vp, _vp)
+_sig("mdsm_rfi_channel_clip", _vp, _u64, _u32, _u64, _u64, _vp, _u32, _vp, _vp)
+_sig("mdsm_rfi_spectrum_clip", _vp, _u64, _u32, _u64, _u64, _vp, _f, _f, _vp, _vp)
+_sig("mdsm_voltage_to_power", _vp, _u64, _u32, _u64, _u64, _vp)
+_sig("mdsm_rfi_clean", _vp, _u64, _u32, _u64, _u64, _P(RfiParams), _vp, _vp, _P(BandpassStats), _vp)
+
+
+def bandpass_fit(bandpass: np.ndarray, params: RfiParams):
+    bp = np.ascontiguousarray(bandpass, np.float64)
+    fitted = np.empty_like(bp)
+    st = BandpassStats()
+    check(_lib.mdsm_bandpass_fit(_ptr(bp), bp.size, C.byref(params), _ptr(fitted), C.byref(st)))
+    return fitted, st.as_dict()
+
+
+def rfi_clean(d_x, ld, nchans, col0, n, params: RfiParams, d_fitted, d_flags=None, stream=None):
+    st = BandpassStats()
+    check(_lib.mdsm_rfi_clean(d_x, ld, nchans, col0, n, C.byref(params), d_fitted, d_flags,
+                              C.byref(st), stream))
+    return st.as_dict()
+

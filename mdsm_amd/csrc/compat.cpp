@@ -56,6 +56,9 @@ struct State {
   mdsm_cand_t *d_cands = nullptr;
   uint32_t *d_count = nullptr;
   uint32_t cap = 1u << 22;
+  double *d_fitted = nullptr;  // fitted bandpass of the beam being cleaned
+  float *d_tmp = nullptr;      // median filter output [tdms][nsamp]
+  std::vector<int32_t> mask_from, mask_to;
   hipStream_t stream = nullptr;
   bool ok = false;
 } g;
@@ -155,6 +158,61 @@ bool detect_beam(unsigned b, const float *d_x, unsigned nsamp_valid, double time
   return true;
 }
 
+// Input conditioning of one beam's rows over [col0, col0 + n): complex
+// voltages to power, then bandpass fit and RFI clipping
+// (multibeam_dedispersion_thread.cu:700-752).  The reference clears the
+// channel flags before channel_clipper runs (:364), so only spectrum
+// clipping changes the data; the flags are not computed here.
+bool clean_input(unsigned b, uint64_t col0, uint64_t n) {
+  SURVEY *s = g.survey;
+  BeamState &bs = g.beams[b];
+  if (n == 0) return true;
+  if (s->voltage && mdsm_voltage_to_power(bs.d_in, g.ld, g.nchans, col0, n, g.stream) != MDSM_OK) {
+    fail("beam %u: %s", b, mdsm_last_error());
+    return false;
+  }
+  if (!s->apply_rfi_clipper) return true;
+  mdsm_rfi_params_t p{};
+  p.ncoeffs = s->ncoeffs;
+  p.channel_block = s->channel_block;
+  p.channel_thresh = s->channel_thresh;
+  p.spectrum_thresh = s->spectrum_thresh;
+  p.nmasks = (uint32_t)g.mask_from.size();
+  p.mask_from = g.mask_from.data();
+  p.mask_to = g.mask_to.data();
+  p.apply_channel_clip = 0;
+  mdsm_bandpass_stats_t st{};
+  if (mdsm_rfi_clean(bs.d_in, g.ld, g.nchans, col0, n, &p, g.d_fitted, nullptr, &st, g.stream) !=
+      MDSM_OK) {
+    fail("beam %u: RFI clipping: %s", b, mdsm_last_error());
+    return false;
+  }
+  s->bandpass_mean = st.mean;
+  s->bandpass_std = st.std;
+  s->bandpass_rmse = st.rmse;
+  return true;
+}
+
+// Median filter and detrending of the dedispersed series (thread.cu:766-772).
+bool filter_output(unsigned b) {
+  SURVEY *s = g.survey;
+  BeamState &bs = g.beams[b];
+  if (s->apply_median_filter) {
+    if (mdsm_median_filter5(bs.d_out, g.tdms, g.nsamp, g.d_tmp, g.stream) != MDSM_OK ||
+        !hip_ok(hipMemcpyAsync(bs.d_out, g.d_tmp, (size_t)g.tdms * g.nsamp * 4,
+                               hipMemcpyDeviceToDevice, g.stream), "median copy")) {
+      fail("beam %u: median filter: %s", b, mdsm_last_error());
+      return false;
+    }
+  }
+  if (s->apply_detrending &&
+      mdsm_detrend_normalise(bs.d_out, g.tdms, g.nsamp, g.stream) != MDSM_OK) {
+    fail("beam %u: detrending: %s", b, mdsm_last_error());
+    return false;
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" const char *mdsm_compat_last_error(void) { return g_err.c_str(); }
@@ -244,8 +302,16 @@ void initialiseMDSM(SURVEY *survey) {
         memset(g.out_ring[k][i], 0, (size_t)g.tdms * g.nsamp * sizeof(float));
   }
   if (!hip_ok(hipMalloc(&g.d_cands, sizeof(mdsm_cand_t) * g.cap), "candidates") ||
-      !hip_ok(hipMalloc(&g.d_count, sizeof(uint32_t)), "count"))
+      !hip_ok(hipMalloc(&g.d_count, sizeof(uint32_t)), "count") ||
+      !hip_ok(hipMalloc(&g.d_fitted, sizeof(double) * g.nchans), "bandpass"))
     return;
+  if (survey->apply_median_filter &&
+      !hip_ok(hipMalloc(&g.d_tmp, (size_t)g.tdms * g.nsamp * sizeof(float)), "median buffer"))
+    return;
+  for (unsigned m = 0; m < survey->num_masks; ++m) {
+    g.mask_from.push_back((int32_t)survey->channel_mask[m].from);
+    g.mask_to.push_back((int32_t)survey->channel_mask[m].to);
+  }
   for (unsigned i = 0; i < survey->nbeams; ++i) open_file(i);
   printf("Observation Params: ndms = %u, maxDM = %f\n", survey->tdms,
          survey->lowdm + survey->dmstep * (survey->tdms - 1));
@@ -262,19 +328,22 @@ float **next_chunk_multibeam(unsigned int data_read, unsigned &samples, double t
                              double blockRate) {
   g.ppnsamp = g.pnsamp;
   g.pnsamp = data_read;
-  g.timestamp = timestamp;
-  g.blockRate = blockRate;
-  if (g.survey) {
-    g.survey->timestamp = timestamp;
-    g.survey->blockRate = blockRate;
-    g.survey->nsamp = data_read;
+  if (g.survey) g.survey->nsamp = data_read;
+  if (data_read > 0) {  // the stop call keeps the last timing (manager.cpp:656-677)
+    g.timestamp = timestamp;
+    g.blockRate = blockRate;
+    if (g.survey) {
+      g.survey->timestamp = timestamp;
+      g.survey->blockRate = blockRate;
+    }
   }
   if (!g.ok || g.loop_counter < 1) {
     samples = (unsigned)-1;
     return nullptr;
   }
   // the series of the chunk submitted two calls earlier (ring slot of chunk
-  // loop_counter - 2); samples is that call's data_read, as in the reference
+  // loop_counter - 2); samples is the data_read of the previous call, as the
+  // reference returns it (manager.cpp:652-653,670,684)
   samples = g.ppnsamp;
   return g.out_ring[g.loop_counter & 1];
 }
@@ -294,7 +363,8 @@ int start_processing(unsigned int data_read) {
     for (unsigned b = 0; b < g.beams.size(); ++b)
       if (!hip_ok(hipMemcpy2DAsync(g.beams[b].d_in + N + (S - n), g.ld * 4,
                                    host + (size_t)b * g.nchans * S, S * 4, n * 4, g.nchans,
-                                   hipMemcpyHostToDevice, g.stream), "priming upload"))
+                                   hipMemcpyHostToDevice, g.stream), "priming upload") ||
+          !clean_input(b, N + (S - n), n))
         return 0;
   } else if (data_read > 0) {
     const int slot = k & 1;
@@ -313,10 +383,12 @@ int start_processing(unsigned int data_read) {
           return 0;
         g.survey->last_nsamp = data_read;
       }
+      if (!clean_input(b, S, data_read)) return 0;
       if (mdsm_dedisperse(bs.plan, bs.d_in, g.ld, bs.d_out, g.stream) != MDSM_OK) {
         fail("beam %u: %s", b, mdsm_last_error());
         return 0;
       }
+      if (!filter_output(b)) return 0;
       if (!hip_ok(hipMemcpyAsync(g.out_ring[slot][b], bs.d_out, (size_t)g.tdms * N * 4,
                                  hipMemcpyDeviceToHost, g.stream), "output download"))
         return 0;
@@ -358,6 +430,8 @@ void tearDownMDSM() {
     }
   if (g.d_cands) (void)hipFree(g.d_cands);
   if (g.d_count) (void)hipFree(g.d_count);
+  if (g.d_fitted) (void)hipFree(g.d_fitted);
+  if (g.d_tmp) (void)hipFree(g.d_tmp);
   if (g.stream) (void)hipStreamDestroy(g.stream);
   printf("Finished Process\n");
   g = State();

@@ -105,6 +105,17 @@ int launch_detrend(float *x, uint64_t nseg, uint32_t L, hipStream_t s);
 int launch_unpack_corner_turn(const void *in, int nbits, uint32_t nspec, uint32_t nchans,
                               uint32_t nbeams, float *out, uint64_t ld, uint64_t beam_stride,
                               uint64_t col0, hipStream_t s);
+int launch_bandpass_sum(const float *x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                        double *out, hipStream_t s);
+int launch_channel_flag(const float *x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                        const double *fitted, uint32_t cb, float thresh, uint8_t *flags,
+                        hipStream_t s);
+int launch_channel_clip(float *x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                        const double *fitted, uint32_t cb, const uint8_t *flags, hipStream_t s);
+int launch_spectrum_clip(float *x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                         const double *fitted, float hi, float lo, uint32_t *count, hipStream_t s);
+int launch_voltage_power(float *x, uint64_t ld, uint32_t nchans, uint64_t col0, uint64_t n,
+                         hipStream_t s);
 int launch_generate(uint64_t seed, uint32_t nchans, int64_t t0, uint64_t n, const float *gain,
                     const float *offset, uint32_t npulses, const int64_t *p_start,
                     const int32_t *p_width, const float *p_amp, const int64_t *p_delay, float *raw,

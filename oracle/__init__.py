@@ -88,6 +88,18 @@ _sig("oracle_process_brute_legacy", _i64, _f32p, _i, _i, _f, _f, _f, _i64, _i32p
 _sig("oracle_detrend_normalise", None, _f32p, _i64, _i, _f32p)
 _sig("oracle_num_threads", _i)
 _sig("oracle_sigproc_unpack", None, _u8p, _i, _i64, _f32p)
+_sig("oracle_bandpass_sum", None, _f32p, _i64, _i, _i64, _i64, _f64p)
+_sig("oracle_bandpass_mask", None, _f64p, _i, _i32p, _i32p, _i)
+_sig("oracle_poly_eval", None, _f64p, _i, _i, _f64p)
+_sig("oracle_bandpass_stats", None, _f64p, _f64p, _i, _f, _f, _f32p)
+_sig("oracle_f64_to_f32_rz", _f, _d)
+_sig("oracle_channel_flagger", None, _f32p, _i64, _i, _i64, _i64, _f64p, _i, _f, _u8p)
+_sig("oracle_channel_clipper", None, _f32p, _i64, _i, _i64, _i64, _f64p, _i, _u8p)
+_sig("oracle_spectrum_clipper", _i64, _f32p, _i64, _i, _i64, _i64, _f64p, _f, _f)
+_sig("oracle_voltage_to_power", None, _f32p, _i64, _i, _i64, _i64)
+_sig("oracle_cnn_param_count", _i64, _i, _i)
+_sig("oracle_cnn_loss_grad", _d, _f64p, _i32p, _i, _i, _i, _f64p, C.c_void_p, C.c_void_p, _i)
+_sig("oracle_bf16_round", _d, _d)
 _sig("oracle_corner_turn", None, _f32p, _i64, _i, _i, _f32p, _i64, _i64, _i64)
 
 KDM = 4148.808
@@ -429,4 +441,124 @@ def ref_read_block(path, byte_offset, nbits, nread):
     out = np.zeros(nread, np.float32)
     got = lib.ref_read_block(path.encode(), byte_offset, nbits, out, nread)
     return out[:got]
+
+
+# ------------------------------------------------------------ bandpass / RFI
+def bandpass_sum(x, col0, n):
+    """Per-channel mean of x[c][col0:col0+n] in the reference's order."""
+    x = _c(x, np.float32)
+    out = np.empty(x.shape[0], np.float64)
+    _lib.oracle_bandpass_sum(x, x.shape[1], x.shape[0], col0, n, out)
+    return out
+
+
+def bandpass_mask(bp, masks):
+    bp = _c(bp, np.float64).copy()
+    if masks:
+        fr = np.array([m[0] for m in masks], np.int32)
+        to = np.array([m[1] for m in masks], np.int32)
+        _lib.oracle_bandpass_mask(bp, bp.size, fr, to, len(masks))
+    return bp
+
+
+def poly_fit(bp, ncoeffs):
+    """Least squares fit of bp[i] by sum_j c_j x_i^j, x_i = i/nchans: SVD
+    least squares (numpy lstsq), the method of gsl_multifit_linear."""
+    n = bp.size
+    xs = np.arange(n) / (1.0 * n)
+    X = np.vander(xs, ncoeffs, increasing=True)
+    return np.linalg.lstsq(X, bp, rcond=None)[0]
+
+
+def poly_eval(coeffs, nchans):
+    coeffs = _c(coeffs, np.float64)
+    out = np.empty(nchans, np.float64)
+    _lib.oracle_poly_eval(coeffs, coeffs.size, nchans, out)
+    return out
+
+
+def bandpass_fit(x, col0, n, ncoeffs=8, masks=(), channel_thresh=10.0, spectrum_thresh=6.0):
+    """The reference's bandpass_fitting + rfi_clipping thresholds:
+    returns (fitted bandpass, dict of mean/std/rmse/thresholds)."""
+    bp = bandpass_mask(bandpass_sum(x, col0, n), list(masks))
+    fitted = poly_eval(poly_fit(bp, ncoeffs), bp.size)
+    return fitted, bandpass_stats(bp, fitted, channel_thresh, spectrum_thresh)
+
+
+def bandpass_stats(bp, fitted, channel_thresh, spectrum_thresh):
+    out = np.empty(6, np.float32)
+    _lib.oracle_bandpass_stats(_c(bp, np.float64), _c(fitted, np.float64), bp.size, channel_thresh,
+                               spectrum_thresh, out)
+    return dict(zip(["mean", "std", "rmse", "channel_thresh", "spectrum_hi", "spectrum_lo"], out.tolist()))
+
+
+def f64_to_f32_rz(d):
+    return np.float32(_lib.oracle_f64_to_f32_rz(float(d)))
+
+
+def channel_flagger(x, col0, n, fitted, cb, thresh):
+    x = _c(x, np.float32)
+    nb = (n + cb - 1) // cb
+    flags = np.empty(x.shape[0] * nb, np.uint8)
+    _lib.oracle_channel_flagger(x, x.shape[1], x.shape[0], col0, n, _c(fitted, np.float64), cb, thresh,
+                                flags)
+    return flags.reshape(x.shape[0], nb)
+
+
+def channel_clipper(x, col0, n, fitted, cb, flags):
+    """Returns a clipped copy of x."""
+    x = _c(x, np.float32).copy()
+    _lib.oracle_channel_clipper(x, x.shape[1], x.shape[0], col0, n, _c(fitted, np.float64), cb,
+                                _c(flags, np.uint8).reshape(-1))
+    return x
+
+
+def spectrum_clipper(x, col0, n, fitted, hi, lo):
+    """Returns (clipped copy of x, number of clipped spectra)."""
+    x = _c(x, np.float32).copy()
+    k = _lib.oracle_spectrum_clipper(x, x.shape[1], x.shape[0], col0, n, _c(fitted, np.float64), hi, lo)
+    return x, int(k)
+
+
+def voltage_to_power(x, col0, n):
+    x = _c(x, np.float32).copy()
+    _lib.oracle_voltage_to_power(x, x.shape[1], x.shape[0], col0, n)
+    return x
+
+
+# ------------------------------------------------------------ triage CNN
+def cnn_param_count(H, W):
+    return int(_lib.oracle_cnn_param_count(H, W))
+
+
+def cnn_loss_grad(x, labels, params, bf16=False, grads=True):
+    """Mean cross-entropy, gradient (flat, same layout as params) and logits
+    of the triage CNN on x[B][2][H][W] (FP64 accumulation)."""
+    x = _c(x, np.float64)
+    B, _, H, W = x.shape
+    params = _c(params, np.float64)
+    assert params.size == cnn_param_count(H, W)
+    g = np.empty_like(params) if grads else None
+    logits = np.empty((B, 2), np.float64)
+    loss = _lib.oracle_cnn_loss_grad(x, _c(labels, np.int32), B, H, W, params,
+                                     g.ctypes.data if grads else None, logits.ctypes.data, 1 if bf16 else 0)
+    return loss, g, logits
+
+
+def bf16_round(v):
+    return _lib.oracle_bf16_round(float(v))
+
+
+def cnn_param_slices(H, W):
+    """Names and [start, stop) of each parameter tensor in the flat layout."""
+    ch = [2, 16, 32, 64]
+    out, off = [], 0
+    for k in range(3):
+        n = ch[k + 1] * ch[k] * 9
+        out.append((f"w{k + 1}", off, off + n)); off += n
+        out.append((f"b{k + 1}", off, off + ch[k + 1])); off += ch[k + 1]
+    fin = 64 * (H // 8) * (W // 8)
+    for name, n in [("fw1", fin * 64), ("fb1", 64), ("fw2", 128), ("fb2", 2)]:
+        out.append((name, off, off + n)); off += n
+    return out
 

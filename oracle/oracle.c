@@ -26,6 +26,8 @@
  *   detrend + normalise         src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:207-295
  *   SIGPROC sample unpack       src/mdsm-multibeam/file_handler.cpp:193-287
  *   corner turn into beams      src/MDSM.cpp:71-104
+ *   bandpass / RFI stage        src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:302-450,
+ *                               src/mdsm-multibeam/multibeam_dedispersion_thread.cu:224-390
  * North-star extensions restated from their own definitions (no reference
  * code exists for them): FP64 round() delay table with 4148.808, per-block
  * FP64 mean/RMS, dyadic boxcar detection, 2-D Chebyshev DBSCAN, FP16/uint8
@@ -47,6 +49,11 @@
  *                      text (the reference needs Qt headers: unbuildable here)
  *   median5            sorted median over random and exhaustive-order inputs
  *   f16 conversion     numpy float16 (round to nearest even)
+ *   bandpass / RFI     exact sums on integer data, float64 means within
+ *                      float rounding, hand-built bursts and spikes whose
+ *                      flagged blocks and clipped spectra are known; the
+ *                      least-squares fit itself is numpy's SVD lstsq (the
+ *                      method of the reference's gsl_multifit_linear)
  *   sigproc unpack     the reference's read_block compiled from its own
  *                      source (oracle/_ref/libref_file.so) and the values
  *                      the fixture writer packed (tests/golden)
@@ -811,5 +818,158 @@ void oracle_corner_turn(const float *spectra, int64_t nspectra, int nchans, int 
     for (int b = 0; b < nbeams; ++b)
       for (int c = 0; c < nchans; ++c)
         out[b * beam_stride + c * ld + col0 + t] = spectra[t * ct + (int64_t)b * nchans + c];
+}
+
+/* ------------------------------------------------------------------ */
+/* Bandpass and RFI stage.  Rows are [nchans][ld]; every function works on
+ * the n samples starting at column col0 of each row (the reference's
+ * `shift`, with `total` = ld).  The reduction orders are those of the
+ * reference kernels with BANDPASS_THREADS = 64 (cache_brute_force.h:16). */
+#define BP_T 64
+
+/* Per-channel mean: lane i sums samples i, i+64, ... in double, then the
+ * halving tree i = 32..1 (bandpass_power_sum, kernel.cu:317-350). */
+void oracle_bandpass_sum(const float *x, int64_t ld, int nchans, int64_t col0, int64_t n,
+                         double *bandpass) {
+  for (int c = 0; c < nchans; ++c) {
+    double p[BP_T];
+    for (int i = 0; i < BP_T; ++i) {
+      p[i] = 0.0;
+      for (int64_t s = i; s < n; s += BP_T) p[i] += x[c * ld + col0 + s];
+    }
+    for (int w = BP_T / 2; w >= 1; w /= 2)
+      for (int i = 0; i < w; ++i) p[i] += p[i + w];
+    bandpass[c] = p[0] / (double)n;
+  }
+}
+
+/* Channel masks (thread.cu:245-258): channels from..to take the mean of
+ * the two neighbouring values, computed in double and stored through a
+ * float. */
+void oracle_bandpass_mask(double *bp, int nchans, const int *from, const int *to, int nmasks) {
+  for (int m = 0; m < nmasks; ++m) {
+    int a = from[m], b = to[m];
+    float value;
+    if (a == 0)
+      value = (float)((bp[b + 1] + bp[b + 2]) * 0.5);
+    else if (b >= nchans - 1)
+      value = (float)((bp[a - 1] + bp[a - 2]) * 0.5);
+    else
+      value = (float)((bp[a - 1] + bp[b + 1]) * 0.5);
+    for (int i = a; i <= b; ++i) bp[i] = value;
+  }
+}
+
+/* Polynomial evaluation at x_i = i/nchans (thread.cu:241,264-270):
+ * fitted[i] = sum_j coeffs[j] * pow(x_i, j), accumulated in j order. */
+void oracle_poly_eval(const double *coeffs, int ncoeffs, int nchans, double *fitted) {
+  for (int i = 0; i < nchans; ++i) {
+    double xi = 0 + i / (1.0 * nchans), acc = 0.0;
+    for (int j = 0; j < ncoeffs; ++j) acc += coeffs[j] * pow(xi, j);
+    fitted[i] = acc;
+  }
+}
+
+/* Bandpass statistics and RFI thresholds (thread.cu:276-296, 344-346):
+ * float accumulators receiving double terms, as the reference's float
+ * SURVEY fields do.  out = {mean, std, rmse, channel_thresh,
+ * spectrum_hi, spectrum_lo}. */
+void oracle_bandpass_stats(const double *bp, const double *fitted, int nchans,
+                           float channel_thresh, float spectrum_thresh, float *out) {
+  float mean = 0, stdv = 0, rmse = 0;
+  for (int i = 0; i < nchans; ++i) mean = (float)(mean + bp[i] / nchans);
+  for (int i = 0; i < nchans; ++i) stdv = (float)(stdv + (bp[i] - mean) * (bp[i] - mean));
+  stdv = sqrtf(stdv / (float)nchans);
+  for (int i = 0; i < nchans; ++i) rmse = (float)(rmse + pow(bp[i] - fitted[i], 2));
+  rmse = sqrtf(rmse / (float)nchans);
+  const float prod_hi = spectrum_thresh * stdv; /* rounded before the add */
+  out[0] = mean;
+  out[1] = stdv;
+  out[2] = rmse;
+  out[3] = channel_thresh * rmse;
+  out[4] = mean + prod_hi;
+  out[5] = mean - prod_hi;
+}
+
+/* float(double) rounded toward zero (__double2float_rz). */
+static float f64_to_f32_rz(double d) {
+  float f = (float)d;
+  if (fabs((double)f) > fabs(d)) f = nextafterf(f, 0.0f);
+  return f;
+}
+
+float oracle_f64_to_f32_rz(double d) { return f64_to_f32_rz(d); }
+
+/* Channel-block flags (channel_flagger, kernel.cu:353-398): the mean of
+ * block b of channel c (lane-strided float sums, halving tree, / cb) above
+ * rz(fitted[c]) + thresh flags blocks b, b-1 (b > 0) and b+1 (b + 2 <
+ * nblocks, the reference's `b < num_blocks - 2`).  Blocks cover
+ * [b*cb, min((b+1)*cb, n)) and a short last block is averaged over its own
+ * samples (the reference reads past the chunk there).  flags:
+ * [nchans][nblocks], cleared first. */
+void oracle_channel_flagger(const float *x, int64_t ld, int nchans, int64_t col0, int64_t n,
+                            const double *fitted, int cb, float thresh, uint8_t *flags) {
+  const int64_t nb = (n + cb - 1) / cb;
+  memset(flags, 0, (size_t)(nchans * nb));
+  for (int c = 0; c < nchans; ++c) {
+    const float bp = f64_to_f32_rz(fitted[c]);
+    for (int64_t b = 0; b < nb; ++b) {
+      float p[BP_T];
+      const int64_t len = (b + 1) * cb <= n ? cb : n - b * cb;
+      for (int i = 0; i < BP_T; ++i) {
+        p[i] = 0.0f;
+        for (int64_t s = i; s < len; s += BP_T) p[i] += x[c * ld + col0 + b * cb + s];
+      }
+      for (int w = BP_T / 2; w >= 1; w /= 2)
+        for (int i = 0; i < w; ++i) p[i] += p[i + w];
+      if (p[0] / (float)len > bp + thresh) {
+        flags[c * nb + b] = 1;
+        if (b > 0) flags[c * nb + b - 1] = 1;
+        if (b + 2 < nb) flags[c * nb + b + 1] = 1;
+      }
+    }
+  }
+}
+
+/* Flagged blocks take rz(fitted[c]) (channel_clipper, kernel.cu:402-423). */
+void oracle_channel_clipper(float *x, int64_t ld, int nchans, int64_t col0, int64_t n,
+                            const double *fitted, int cb, const uint8_t *flags) {
+  const int64_t nb = (n + cb - 1) / cb;
+  for (int c = 0; c < nchans; ++c) {
+    const float bp = f64_to_f32_rz(fitted[c]);
+    for (int64_t b = 0; b < nb; ++b) {
+      if (!flags[c * nb + b]) continue;
+      for (int64_t s = b * cb; s < (b + 1) * cb && s < n; ++s) x[c * ld + col0 + s] = bp;
+    }
+  }
+}
+
+/* Spectra whose channel mean (sequential float sum / nchans) lies above hi
+ * or below lo are replaced by rz(fitted[c]) (spectrum_clipper,
+ * kernel.cu:427-450).  Returns the number of clipped spectra. */
+int64_t oracle_spectrum_clipper(float *x, int64_t ld, int nchans, int64_t col0, int64_t n,
+                                const double *fitted, float hi, float lo) {
+  int64_t clipped = 0;
+  for (int64_t s = 0; s < n; ++s) {
+    float m = 0.0f;
+    for (int c = 0; c < nchans; ++c) m += x[c * ld + col0 + s];
+    m /= (float)nchans;
+    if (m > hi || m < lo) {
+      for (int c = 0; c < nchans; ++c) x[c * ld + col0 + s] = f64_to_f32_rz(fitted[c]);
+      ++clipped;
+    }
+  }
+  return clipped;
+}
+
+/* Complex 16-bit voltages (re, im) stored in each 32-bit slot become their
+ * power re^2 + im^2 (voltage_to_power, kernel.cu:302-312). */
+void oracle_voltage_to_power(float *x, int64_t ld, int nchans, int64_t col0, int64_t n) {
+  for (int c = 0; c < nchans; ++c)
+    for (int64_t s = 0; s < n; ++s) {
+      int16_t v[2];
+      memcpy(v, x + c * ld + col0 + s, 4);
+      x[c * ld + col0 + s] = (float)(v[0] * v[0] + v[1] * v[1]);
+    }
 }
 

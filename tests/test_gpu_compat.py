@@ -55,7 +55,7 @@ def test_streaming_manager_matches_one_shot(tmp_path):
                         pulses=[(20.0, 6000, 8, 1.0), (31.0, 15000, 4, 1.5)])
     # one-shot reference over the whole stream
     full = O.dedisperse_f32seq(stream, table, total - S)
-    got = {}
+    got, reads = {}, []
     for i in range(nchunks + 3):
         buf = lib.mdsm_compat_buffer()
         if i == 0:
@@ -69,17 +69,23 @@ def test_streaming_manager_matches_one_shot(tmp_path):
         else:
             n = 0
         samples = C.c_uint()
+        reads.append(n)
         outs = lib.mdsm_compat_next_chunk(n, C.byref(samples), 64e-6 * i * N, 64e-6)
+        if i < 1:
+            assert not outs and samples.value == 0xFFFFFFFF
         if i >= 3 and i - 2 <= nchunks:
             arr = np.ctypeslib.as_array(outs[0], shape=(M, N)).copy()
+            # the reference reports the previous call's data_read with the
+            # buffer of the chunk two calls back (0 for the last chunk)
+            assert samples.value == reads[i - 1]
             got[i - 2] = (arr, samples.value)
         r = lib.mdsm_compat_start(n)
         if r == 0:
             break
     lib.mdsm_compat_close()
     assert sorted(got) == list(range(1, nchunks + 1))
+    assert [got[k][1] for k in sorted(got)] == [N] * (nchunks - 1) + [0]
     for k, (arr, samples) in got.items():
-        assert samples == N
         assert np.array_equal(arr, full[:, (k - 1) * N:k * N]), k
     lines = (tmp_path / "compat.dat").read_text().strip().splitlines()
     assert lines, "no candidates written"
@@ -88,3 +94,75 @@ def test_streaming_manager_matches_one_shot(tmp_path):
     # both pulses appear as clusters near their DM
     dms = {round(r[1]) for r in rows if r[3] > 0}
     assert any(abs(d - 20) <= 2 for d in dms) and any(abs(d - 31) <= 2 for d in dms), dms
+
+
+XML_RFI = """<observation>
+  <dm lowDM="0" numDMs="24" dmStep="1.0" />
+  <channels nchans="64" npols="1" ncoeffs="6" />
+  <timing tsamp="0.000064" />
+  <samples nsamp="4096" nbits="32" />
+  <rfi applyRFIClipper="1" channelBlock="1024" spectrumThreshold="4" channelThreshold="10" channelMask="5,20-22"/>
+  <detection detectionThreshold="6" applyMedianFilter="1" applyDetrending="1" />
+  <writer filePrefix="rfi" baseDirectory="{dir}" singleFileMode="1" />
+  <beams><beam beamId="0" topFrequency="1500" frequencyOffset="-1.0" /></beams>
+</observation>"""
+
+
+def _clean(block, masks):
+    """The oracle's bandpass fit + spectrum clipping of one block (the
+    reference clears the channel flags before clipping them)."""
+    fitted, st = O.bandpass_fit(block, 0, block.shape[1], 6, masks, 10.0, 4.0)
+    out, k = O.spectrum_clipper(block, 0, block.shape[1], fitted, st["spectrum_hi"], st["spectrum_lo"])
+    return out, k
+
+
+def test_streaming_manager_rfi_median_detrend(tmp_path):
+    xml = tmp_path / "obs.xml"
+    xml.write_text(XML_RFI.format(dir=str(tmp_path)))
+    assert lib.mdsm_compat_open(str(xml).encode(), 0) == 0, lib.mdsm_compat_last_error()
+    nb, nch, ns, nd, ms = (C.c_uint() for _ in range(5))
+    lib.mdsm_compat_survey(C.byref(nb), C.byref(nch), C.byref(ns), C.byref(nd), C.byref(ms))
+    C_, N, M, S = nch.value, ns.value, nd.value, ms.value
+    print(f"[compat rfi] C={C_} N={N} M={M} maxshift={S}", file=sys.stderr, flush=True)
+    table = O.delay_table_ref(1500.0, -1.0, C_, 64e-6, 0.0, 1.0, M)
+    nchunks = 3
+    total = S + nchunks * N
+    gain, offset = O.channel_bandpass(5, C_)  # a bandpass to fit: gains 0.5-2, offsets 0-10
+    stream = O.generate(5, C_, 1500.0, -1.0, 64e-6, 0, total, gain=gain, offset=offset,
+                        pulses=[(12.0, 5000, 8, 2.0)])
+    stream[:, S + 1000] += 40.0  # broadband RFI in chunk 1
+    stream[:, S + N + 300:S + N + 303] += 60.0  # and in chunk 2
+    masks = [(5, 5), (20, 22)]
+    cleaned = [_clean(stream[:, :S], masks)[0]]
+    clipped = 0
+    for k in range(nchunks):
+        block, n = _clean(stream[:, S + k * N:S + (k + 1) * N], masks)
+        cleaned.append(block)
+        clipped += n
+    assert clipped >= 4
+    cleaned = np.concatenate(cleaned, axis=1)
+    full = O.dedisperse_f32seq(cleaned, table, total - S)
+    got = {}
+    for i in range(nchunks + 3):
+        buf = lib.mdsm_compat_buffer()
+        if i == 0:
+            n = S
+            np.ctypeslib.as_array(buf, shape=(C_, S))[:] = stream[:, :S]
+        elif i <= nchunks:
+            n = N
+            np.ctypeslib.as_array(buf, shape=(C_, N))[:] = stream[:, S + (i - 1) * N:S + i * N]
+        else:
+            n = 0
+        samples = C.c_uint()
+        outs = lib.mdsm_compat_next_chunk(n, C.byref(samples), 64e-6 * i * N, 64e-6)
+        if i >= 3 and i - 2 <= nchunks:
+            got[i - 2] = np.ctypeslib.as_array(outs[0], shape=(M, N)).copy()
+        if lib.mdsm_compat_start(n) == 0:
+            break
+    lib.mdsm_compat_close()
+    assert sorted(got) == list(range(1, nchunks + 1))
+    for k, arr in got.items():
+        series = full[:, (k - 1) * N:k * N]
+        want = O.detrend_normalise(O.median_filter5(series), N).reshape(M, N)
+        # replacement values come from two least-squares solvers (1 ulp at most)
+        np.testing.assert_allclose(arr, want, rtol=1e-4, atol=1e-4, err_msg=f"chunk {k}")

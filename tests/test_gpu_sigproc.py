@@ -77,7 +77,7 @@ def test_reader_streams_file_in_chunks(tmp_path):
     data = rng.integers(0, 16, (nspec, nchans * nbeams))
     path = tmp_path / "stream.fil"
     path.write_bytes(mkfil.header(nchans * nbeams, nbits) + mkfil.pack(data, nbits))
-    ld = nspec + 32
+    ld = nspec + 1500 + 32  # every request must fit its row, even past the end of the file
     d_out = abi.DeviceBuffer(4 * nbeams * nchans * ld)
     d_out.zero()
     reader = abi.SigprocReader(str(path), nchans=0, nbeams=nbeams, max_spectra=1500)
