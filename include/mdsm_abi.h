@@ -313,6 +313,38 @@ int mdsm_rfi_clean(float *d_x, uint64_t ld, uint32_t nchans, uint64_t col0, uint
                    const mdsm_rfi_params_t *params, double *d_fitted, uint8_t *d_flags,
                    mdsm_bandpass_stats_t *stats, void *stream);
 
+/* ------------------------------------------------------- triage network
+ * A BF16 CNN that classifies candidate events from paired snippets (DM-time
+ * and dedispersed subband-time, each H x W) as pulse (class 1) or not
+ * (class 0).  It replaces the reference's heuristic cluster triage
+ * (Cluster::computeTransientProbability, dbscan.cpp:25-171), which stays
+ * available as mdsm_dbscan_ref + classification on the host.
+ *   conv 3x3 (pad 1) 2->16->32->64, each + bias, ReLU, 2x2 max pool;
+ *   fc 64*(H/8)*(W/8) -> 64 (ReLU) -> 2 logits; softmax cross-entropy.
+ * Products run on MFMA with BF16 operands and FP32 accumulation; master
+ * weights, biases and Adam moments are FP32.
+ * Parameter arrays (host, float) use the canonical layout w1 b1 w2 b2 w3 b3
+ * fw1 fb1 fw2 fb2: conv weights [co][ci][ky][kx], fw1[o][c*(H/8)*(W/8) +
+ * y*(W/8) + x], fw2[o][i]. */
+typedef struct mdsm_cnn_s *mdsm_cnn;
+int mdsm_cnn_create(int height, int width, int max_batch, mdsm_cnn *net);
+int mdsm_cnn_destroy(mdsm_cnn net);
+int mdsm_cnn_param_count(mdsm_cnn net, uint64_t *count);
+int mdsm_cnn_set_params(mdsm_cnn net, const float *host_params); /* also resets Adam */
+int mdsm_cnn_get_params(mdsm_cnn net, float *host_params);
+int mdsm_cnn_get_grads(mdsm_cnn net, float *host_grads);    /* of the last backward */
+int mdsm_cnn_init(mdsm_cnn net, uint64_t seed);              /* He-normal weights, zero biases */
+/* d_x: BF16 bits [batch][2][H][W]; d_logits (may be NULL): float [batch][2]. */
+int mdsm_cnn_forward(mdsm_cnn net, const uint16_t *d_x, uint32_t batch, float *d_logits,
+                     void *stream);
+/* Gradients of the mean cross-entropy of the last forward; d_labels int32
+ * [batch] in {0,1}; d_loss (may be NULL) receives the loss. */
+int mdsm_cnn_backward(mdsm_cnn net, const int32_t *d_labels, uint32_t batch, float *d_loss,
+                      void *stream);
+int mdsm_cnn_adam_step(mdsm_cnn net, float lr, float weight_decay, void *stream);
+int mdsm_cnn_train_step(mdsm_cnn net, const uint16_t *d_x, const int32_t *d_labels,
+                        uint32_t batch, float lr, float *d_loss, void *stream);
+
 /* ------------------------------------------------------- SIGPROC ingest
  * Filterbank files: a keyword header (read_header,
  * src/mdsm-multibeam/file_handler.cpp:56-190) followed by time-major

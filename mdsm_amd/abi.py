@@ -623,3 +623,80 @@ def rfi_clean(d_x, ld, nchans, col0, n, params: RfiParams, d_fitted, d_flags=Non
                               C.byref(st), stream))
     return st.as_dict()
 
+
+# ------------------------------------------------------------ triage network
+_sig("mdsm_cnn_create", _i, _i, _i, _P(_vp))
+_sig("mdsm_cnn_destroy", _vp)
+_sig("mdsm_cnn_param_count", _vp, _P(_u64))
+_sig("mdsm_cnn_set_params", _vp, _vp)
+_sig("mdsm_cnn_get_params", _vp, _vp)
+_sig("mdsm_cnn_get_grads", _vp, _vp)
+_sig("mdsm_cnn_init", _vp, C.c_uint64)
+_sig("mdsm_cnn_forward", _vp, _vp, _u32, _vp, _vp)
+_sig("mdsm_cnn_backward", _vp, _vp, _u32, _vp, _vp)
+_sig("mdsm_cnn_adam_step", _vp, _f, _f, _vp)
+_sig("mdsm_cnn_train_step", _vp, _vp, _vp, _u32, _f, _vp, _vp)
+
+
+def to_bf16_bits(x) -> np.ndarray:
+    """float32 -> BF16 bit patterns (round to nearest even)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bf16_bits(h) -> np.ndarray:
+    return (np.asarray(h, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+class TriageNet:
+    """The BF16 triage CNN (forward, backward and Adam on the device)."""
+
+    def __init__(self, height=64, width=256, max_batch=256):
+        self.handle = _vp()
+        check(_lib.mdsm_cnn_create(height, width, max_batch, C.byref(self.handle)))
+        n = _u64()
+        check(_lib.mdsm_cnn_param_count(self.handle, C.byref(n)))
+        self.nparams = n.value
+        self.H, self.W, self.max_batch = height, width, max_batch
+
+    def init(self, seed):
+        check(_lib.mdsm_cnn_init(self.handle, seed))
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        check(_lib.mdsm_cnn_set_params(self.handle, _ptr(p)))
+
+    def params(self):
+        out = np.empty(self.nparams, np.float32)
+        check(_lib.mdsm_cnn_get_params(self.handle, _ptr(out)))
+        return out
+
+    def grads(self):
+        out = np.empty(self.nparams, np.float32)
+        check(_lib.mdsm_cnn_get_grads(self.handle, _ptr(out)))
+        return out
+
+    def forward(self, d_x, batch, d_logits=None, stream=None):
+        check(_lib.mdsm_cnn_forward(self.handle, d_x, batch, d_logits, stream))
+
+    def backward(self, d_labels, batch, d_loss=None, stream=None):
+        check(_lib.mdsm_cnn_backward(self.handle, d_labels, batch, d_loss, stream))
+
+    def adam_step(self, lr, weight_decay=0.0, stream=None):
+        check(_lib.mdsm_cnn_adam_step(self.handle, lr, weight_decay, stream))
+
+    def train_step(self, d_x, d_labels, batch, lr, d_loss=None, stream=None):
+        check(_lib.mdsm_cnn_train_step(self.handle, d_x, d_labels, batch, lr, d_loss, stream))
+
+    def close(self):
+        if self.handle:
+            check(_lib.mdsm_cnn_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
