@@ -330,6 +330,7 @@ typedef struct mdsm_cnn_s *mdsm_cnn;
 int mdsm_cnn_create(int height, int width, int max_batch, mdsm_cnn *net);
 int mdsm_cnn_destroy(mdsm_cnn net);
 int mdsm_cnn_param_count(mdsm_cnn net, uint64_t *count);
+int mdsm_cnn_shape(mdsm_cnn net, int *height, int *width, int *max_batch);
 int mdsm_cnn_set_params(mdsm_cnn net, const float *host_params); /* also resets Adam */
 int mdsm_cnn_get_params(mdsm_cnn net, float *host_params);
 int mdsm_cnn_get_grads(mdsm_cnn net, float *host_grads);    /* of the last backward */
@@ -344,6 +345,86 @@ int mdsm_cnn_backward(mdsm_cnn net, const int32_t *d_labels, uint32_t batch, flo
 int mdsm_cnn_adam_step(mdsm_cnn net, float lr, float weight_decay, void *stream);
 int mdsm_cnn_train_step(mdsm_cnn net, const uint16_t *d_x, const int32_t *d_labels,
                         uint32_t batch, float lr, float *d_loss, void *stream);
+
+/* ------------------------------------------------------- triage data
+ * Synthetic training and evaluation streams for the triage network, made by
+ * the plain search (normalise, dedisperse, per-block statistics, boxcar
+ * detection, 2-D DBSCAN, cluster peaks) over seeded chunks with injected
+ * dispersed pulses and RFI: zero-DM broadband spikes, narrowband bursts,
+ * 50 Hz-modulated channels and on/off ("telegraph") channels. */
+enum { MDSM_RFI_BURST = 0, MDSM_RFI_MAINS = 1, MDSM_RFI_TELEGRAPH = 2 };
+typedef struct {
+  uint32_t chan;
+  int32_t kind;     /* MDSM_RFI_* */
+  int64_t start;    /* first sample (row index) */
+  int64_t len;      /* samples */
+  float amp;        /* in units of the channel's noise sigma (scaled by its gain) */
+  float period;     /* MAINS: samples per cycle; TELEGRAPH: samples per on/off block */
+} mdsm_rfi_item_t;
+/* Adds the items to raw rows [nchans][ld] (columns [0, n)) on the device:
+ * BURST amp; MAINS amp*(1+sin(2 pi t/period))/2; TELEGRAPH amp in blocks
+ * chosen with probability 1/4 by Philox4x32-10 from (seed, block, chan). */
+int mdsm_inject_rfi(const mdsm_rfi_item_t *host_items, uint32_t nitems, const float *d_gain,
+                    uint64_t seed, float *d_raw, uint64_t ld, uint64_t n, void *stream);
+
+typedef struct {
+  mdsm_survey_t survey;        /* nsamp = output samples per chunk */
+  uint32_t npulses;            /* injected pulses per chunk */
+  double dm_lo, dm_hi;         /* pulse DM, uniform */
+  double width_lo, width_hi;   /* pulse width in samples, log-uniform */
+  double snr_lo, snr_hi;       /* matched-boxcar S/N, uniform */
+  uint32_t n_spikes, n_bursts, n_mains, n_telegraph; /* RFI items per chunk */
+  double rfi_lo, rfi_hi;       /* RFI strength (spike S/N; narrowband amplitude in sigma) */
+  double thresh;               /* detection S/N (6) */
+  uint32_t stats_block;        /* samples per statistics block (16384) */
+  int32_t nwidths;             /* boxcar widths (7) */
+  int32_t eps, minpts;         /* DBSCAN (3, 4) */
+  int32_t snip_dms, snip_samples, snip_subbands; /* snippet shape (64, 256, 64) */
+} mdsm_triage_config_t;
+typedef struct {
+  int32_t dm_idx, sample, width_log2;  /* cluster peak */
+  float snr;
+  int32_t cluster_size;
+  int32_t label;                       /* 1: matches an injected pulse */
+  int32_t pulse;                       /* index of that pulse in its chunk, else -1 */
+  int32_t chunk;
+} mdsm_event_t;
+typedef struct {
+  uint64_t chunks, injected, injected_found; /* pulses injected / matched by >= 1 cluster */
+  uint64_t clusters, false_clusters;          /* all clusters / clusters matching no pulse */
+  uint64_t candidates;                        /* threshold crossings before clustering */
+} mdsm_triage_stats_t;
+/* Snippets of nev events (host arrays: DM index, centre sample) from a
+ * beam's dedispersed series d_out [ndms][nsamp] and its normalised FP32
+ * input d_store [nchans][ld] with the plan's delay table d_table
+ * [nchans][ndms]: plane 0 = out rows dm-SD/2 .. dm+SD/2-1 times scale0,
+ * plane 1 = SB subband sums (channels in order) of the input dedispersed at
+ * the event DM times scale1, both over samples tc-ST/2 .. tc+ST/2-1, zero
+ * outside the arrays, clipped to +-clip, stored as BF16 [nev][2][SD][ST]
+ * (SB == SD). */
+int mdsm_triage_snippets(const float *d_out, uint32_t ndms, uint32_t nsamp, const float *d_store,
+                         uint64_t ld, uint32_t nchans, const int32_t *d_table, const int32_t *ev_dm,
+                         const int32_t *ev_t, uint32_t nev, int SD, int ST, int SB, float scale0,
+                         float scale1, float clip, uint16_t *d_snippets, void *stream);
+typedef struct mdsm_triage_sim_s *mdsm_triage_sim;
+int mdsm_triage_sim_create(const mdsm_triage_config_t *config, uint64_t seed, mdsm_triage_sim *sim);
+int mdsm_triage_sim_destroy(mdsm_triage_sim sim);
+/* Runs chunks until nchunks are done or max_events events are stored.
+ * Snippets go to d_snippets (BF16 [max_events][2][snip_dms or
+ * snip_subbands][snip_samples]; both planes have snip_dms rows, so
+ * snip_subbands must equal snip_dms) from index *n_events on; events to
+ * host_events; *n_events is advanced.  Stats accumulate. */
+int mdsm_triage_sim_run(mdsm_triage_sim sim, uint32_t nchunks, uint16_t *d_snippets,
+                        mdsm_event_t *host_events, uint32_t max_events, uint32_t *n_events,
+                        mdsm_triage_stats_t *stats);
+/* One pass over n snippets in a seeded random order, batches of `batch`;
+ * *mean_loss receives the mean of the batch losses. */
+int mdsm_cnn_train_epoch(mdsm_cnn net, const uint16_t *d_snippets, const int32_t *d_labels,
+                         uint32_t n, uint32_t batch, float lr, uint64_t seed, float *mean_loss,
+                         void *stream);
+/* Pulse probability (softmax class 1) of n snippets, device float[n]. */
+int mdsm_cnn_predict(mdsm_cnn net, const uint16_t *d_snippets, uint32_t n, float *d_prob,
+                     void *stream);
 
 /* ------------------------------------------------------- SIGPROC ingest
  * Filterbank files: a keyword header (read_header,

@@ -700,3 +700,104 @@ class TriageNet:
         except Exception:
             pass
 
+
+# ------------------------------------------------------------ triage data
+class RfiItem(C.Structure):
+    _fields_ = [("chan", C.c_uint32), ("kind", C.c_int32), ("start", C.c_int64), ("len", C.c_int64),
+                ("amp", C.c_float), ("period", C.c_float)]
+
+
+RFI_BURST, RFI_MAINS, RFI_TELEGRAPH = 0, 1, 2
+
+
+class TriageConfig(C.Structure):
+    _fields_ = [("survey", Survey), ("npulses", C.c_uint32), ("dm_lo", _d), ("dm_hi", _d),
+                ("width_lo", _d), ("width_hi", _d), ("snr_lo", _d), ("snr_hi", _d),
+                ("n_spikes", C.c_uint32), ("n_bursts", C.c_uint32), ("n_mains", C.c_uint32),
+                ("n_telegraph", C.c_uint32), ("rfi_lo", _d), ("rfi_hi", _d), ("thresh", _d),
+                ("stats_block", C.c_uint32), ("nwidths", C.c_int32), ("eps", C.c_int32),
+                ("minpts", C.c_int32), ("snip_dms", C.c_int32), ("snip_samples", C.c_int32),
+                ("snip_subbands", C.c_int32)]
+
+    @classmethod
+    def make(cls, survey, npulses=20, dm=(10.0, 800.0), width=(1.0, 64.0), snr=(6.0, 30.0),
+             n_spikes=8, n_bursts=6, n_mains=2, n_telegraph=2, rfi=(8.0, 40.0), thresh=6.0,
+             stats_block=16384, nwidths=7, eps=3, minpts=4, snip=(64, 256)):
+        return cls(survey, npulses, dm[0], dm[1], width[0], width[1], snr[0], snr[1], n_spikes, n_bursts,
+                   n_mains, n_telegraph, rfi[0], rfi[1], thresh, stats_block, nwidths, eps, minpts,
+                   snip[0], snip[1], snip[0])
+
+
+class TriageStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("chunks", "injected", "injected_found", "clusters",
+                                          "false_clusters", "candidates")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EVENT_DTYPE = np.dtype([("dm_idx", "<i4"), ("sample", "<i4"), ("width_log2", "<i4"), ("snr", "<f4"),
+                        ("cluster_size", "<i4"), ("label", "<i4"), ("pulse", "<i4"), ("chunk", "<i4")])
+
+_sig("mdsm_inject_rfi", _vp, _u32, _vp, C.c_uint64, _vp, _u64, _u64, _vp)
+_sig("mdsm_triage_snippets", _vp, _u32, _u32, _vp, _u64, _u32, _vp, _vp, _vp, _u32, _i, _i, _i, _f, _f,
+     _f, _vp, _vp)
+_sig("mdsm_triage_sim_create", _P(TriageConfig), C.c_uint64, _P(_vp))
+_sig("mdsm_triage_sim_destroy", _vp)
+_sig("mdsm_triage_sim_run", _vp, _u32, _vp, _vp, _u32, _P(_u32), _P(TriageStats))
+_sig("mdsm_cnn_train_epoch", _vp, _vp, _vp, _u32, _u32, _f, C.c_uint64, _P(_f), _vp)
+_sig("mdsm_cnn_predict", _vp, _vp, _u32, _vp, _vp)
+
+
+def inject_rfi(items, d_gain, seed, d_raw, ld, n, stream=None):
+    arr = (RfiItem * len(items))(*items)
+    check(_lib.mdsm_inject_rfi(C.cast(arr, _vp), len(items), d_gain, seed, d_raw, ld, n, stream))
+
+
+def triage_snippets(d_out, ndms, nsamp, d_store, ld, nchans, d_table, ev_dm, ev_t, d_snip, SD=64, ST=256,
+                    SB=64, scale0=1.0, scale1=1.0, clip=20.0, stream=None):
+    ev_dm = np.ascontiguousarray(ev_dm, np.int32)
+    ev_t = np.ascontiguousarray(ev_t, np.int32)
+    check(_lib.mdsm_triage_snippets(d_out, ndms, nsamp, d_store, ld, nchans, d_table, _ptr(ev_dm),
+                                    _ptr(ev_t), ev_dm.size, SD, ST, SB, scale0, scale1, clip, d_snip,
+                                    stream))
+
+
+class TriageSim:
+    """Seeded synthetic search stream producing labelled events + snippets."""
+
+    def __init__(self, config: TriageConfig, seed: int):
+        self.config = config
+        self.handle = _vp()
+        check(_lib.mdsm_triage_sim_create(C.byref(config), seed, C.byref(self.handle)))
+        self.stats = TriageStats()
+
+    def run(self, nchunks, d_snip, max_events, events=None, n_events=0):
+        if events is None:
+            events = np.zeros(max_events, EVENT_DTYPE)
+        n = _u32(n_events)
+        check(_lib.mdsm_triage_sim_run(self.handle, nchunks, d_snip, _ptr(events), max_events, C.byref(n),
+                                       C.byref(self.stats)))
+        return events, n.value
+
+    def close(self):
+        if self.handle:
+            check(_lib.mdsm_triage_sim_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def train_epoch(net, d_snip, d_labels, n, batch, lr, seed, stream=None):
+    loss = _f()
+    check(_lib.mdsm_cnn_train_epoch(net.handle, d_snip, d_labels, n, batch, lr, seed, C.byref(loss), stream))
+    return loss.value
+
+
+def predict(net, d_snip, n, d_prob, stream=None):
+    check(_lib.mdsm_cnn_predict(net.handle, d_snip, n, d_prob, stream))
+

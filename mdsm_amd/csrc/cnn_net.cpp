@@ -170,6 +170,14 @@ extern "C" int mdsm_cnn_param_count(mdsm_cnn n, uint64_t *count) {
   return MDSM_OK;
 }
 
+extern "C" int mdsm_cnn_shape(mdsm_cnn n, int *height, int *width, int *max_batch) {
+  MDSM_REQUIRE(n, MDSM_ERR_ARG, "null network");
+  if (height) *height = n->H;
+  if (width) *width = n->W;
+  if (max_batch) *max_batch = n->Bmax;
+  return MDSM_OK;
+}
+
 extern "C" int mdsm_cnn_set_params(mdsm_cnn n, const float *host) {
   MDSM_REQUIRE(n && host, MDSM_ERR_ARG, "null argument");
   const auto map = device_index(n);

@@ -100,6 +100,8 @@ _sig("oracle_voltage_to_power", None, _f32p, _i64, _i, _i64, _i64)
 _sig("oracle_cnn_param_count", _i64, _i, _i)
 _sig("oracle_cnn_loss_grad", _d, _f64p, _i32p, _i, _i, _i, _f64p, C.c_void_p, C.c_void_p, _i)
 _sig("oracle_bf16_round", _d, _d)
+_sig("oracle_rfi_inject", None, C.c_void_p, _i, _f32p, _u64, _f32p, _i64, _i64)
+_sig("oracle_snippet", None, _f32p, _i, _i, _f32p, _i64, _i, _i32p, _i, _i, _i, _i, _i, _f, _f, _f, _f32p)
 _sig("oracle_corner_turn", None, _f32p, _i64, _i, _i, _f32p, _i64, _i64, _i64)
 
 KDM = 4148.808
@@ -561,4 +563,29 @@ def cnn_param_slices(H, W):
     for name, n in [("fw1", fin * 64), ("fb1", 64), ("fw2", 128), ("fb2", 2)]:
         out.append((name, off, off + n)); off += n
     return out
+
+
+# ------------------------------------------------------------ triage data
+RFI_ITEM_DTYPE = np.dtype([("chan", "<u4"), ("kind", "<i4"), ("start", "<i8"), ("len", "<i8"),
+                           ("amp", "<f4"), ("period", "<f4")])
+
+
+def rfi_inject(raw, items, gain, seed):
+    """Returns a copy of raw [nchans][ld] with the RFI items added."""
+    raw = _c(raw, np.float32).copy()
+    items = np.ascontiguousarray(items, RFI_ITEM_DTYPE)
+    _lib.oracle_rfi_inject(items.ctypes.data, items.size, _c(gain, np.float32), seed, raw, raw.shape[1],
+                           raw.shape[1])
+    return raw
+
+
+def snippet(out, store, table, dm0, tc, SD=64, ST=256, SB=64, scale0=1.0, scale1=1.0, clip=20.0):
+    """The two snippet planes [2][SD][ST] of one event (BF16 values)."""
+    out = _c(out, np.float32)
+    store = _c(store, np.float32)
+    table = _c(table, np.int32)
+    res = np.zeros((2, SD, ST), np.float32)
+    _lib.oracle_snippet(out, out.shape[0], out.shape[1], store, store.shape[1], store.shape[0], table,
+                        dm0, tc, SD, ST, SB, scale0, scale1, clip, res)
+    return res
 
