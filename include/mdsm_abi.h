@@ -202,6 +202,17 @@ int mdsm_cluster_peaks(const mdsm_cand_t *cands, const int32_t *labels, uint32_t
 int mdsm_dbscan_ref(const double *time, const double *dm, const double *snr, uint32_t n,
                     float min_time, float min_dm, float min_snr, uint32_t min_points,
                     int32_t *cluster, uint32_t *nclusters);
+/* Reference cluster classification (Cluster::computeTransientProbability,
+ * dbscan.cpp:25-171) over the n member points of one cluster, with the
+ * survey's DM grid, minimum pulse width (ms) and band (beam 0's fch1, foff,
+ * nchans).  *score: 1 when a rule rejects the cluster (peak DM <= 2, pulse
+ * width not above min_pulse_width), else the mean squared distance of the
+ * smoothed DM-S/N curve to the smearing model; the reference keeps clusters
+ * with score < 0.1 (multibeam_dedispersion_output.cpp:262-287).  info
+ * (may be NULL) receives {peak DM, peak S/N, width ms, centre time}. */
+int mdsm_classify_cluster_ref(const double *time, const double *dm, const double *snr, uint32_t n,
+                              float min_dm, float dm_step, uint32_t num_dms, float min_pulse_width,
+                              float fch1, float foff, uint32_t nchans, float *score, double *info);
 
 /* ---------------------------------------------------------- multi-beam batch
  * Many beams of uneven nchans/nsamp/ndms dedispersed by one persistent

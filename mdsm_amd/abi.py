@@ -109,6 +109,7 @@ _sig("mdsm_sort_candidates", _vp, _u32)
 _sig("mdsm_dbscan2d", _vp, _u32, _i, _i, _vp, _P(_u32))
 _sig("mdsm_cluster_peaks", _vp, _vp, _u32, _u32, _vp, _vp)
 _sig("mdsm_dbscan_ref", _vp, _vp, _vp, _u32, _f, _f, _f, _u32, _vp, _P(_u32))
+_sig("mdsm_classify_cluster_ref", _vp, _vp, _vp, _u32, _f, _f, _u32, _f, _f, _f, _u32, _P(_f), _vp)
 _sig("mdsm_batch_create", _P(Survey), _u32, _i, _P(_vp))
 _sig("mdsm_batch_destroy", _vp)
 _sig("mdsm_batch_info", _vp, _u32, _P(_u32), _P(_u64))
@@ -369,6 +370,16 @@ def dbscan_ref(time, dm, snr, min_time, min_dm, min_snr, min_points):
 
 
 # ------------------------------------------------------------------ batch
+def classify_cluster_ref(time, dm, snr, min_dm, dm_step, num_dms, min_pulse_width, fch1, foff, nchans):
+    """Reference cluster classification (score, [maxDM, maxSNR, width, position])."""
+    t, d, s = (np.ascontiguousarray(a, np.float64) for a in (time, dm, snr))
+    score = _f()
+    info = np.zeros(4, np.float64)
+    check(_lib.mdsm_classify_cluster_ref(_ptr(t), _ptr(d), _ptr(s), t.size, min_dm, dm_step, num_dms,
+                                         min_pulse_width, fch1, foff, nchans, C.byref(score), _ptr(info)))
+    return score.value, info
+
+
 class Batch:
     def __init__(self, surveys, dtype=F32):
         arr = (Survey * len(surveys))(*surveys)

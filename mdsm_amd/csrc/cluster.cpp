@@ -9,6 +9,7 @@
 // from the neighbour count, float comparisons, ids from 1, noise -1) so that
 // the compatibility path clusters exactly as MDSM did.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -150,5 +151,83 @@ extern "C" int mdsm_dbscan_ref(const double *tm, const double *dm, const double 
     }
   }
   if (nclusters) *nclusters = (uint32_t)nc;
+  return MDSM_OK;
+}
+
+// Cluster classification (Cluster::computeTransientProbability,
+// src/mdsm-multibeam/dbscan.cpp:25-171) with the reference's float/double
+// arithmetic: a DM-S/N histogram, its (quirky) 3-tap smoothing, the median
+// DM of the bins within 0.9 of the peak, rejection rules (DM <= 2, invalid
+// width) and the mean squared distance to the dispersion-smearing model
+// sqrt(pi)/2 * erf(y)/y.  The reference keeps clusters scoring < 0.1.
+extern "C" int mdsm_classify_cluster_ref(const double *tm, const double *dm, const double *snr,
+                                         uint32_t n, float min_dm, float dm_step, uint32_t num_dms,
+                                         float min_pulse_width, float fch1, float foff,
+                                         uint32_t nchans, float *score, double *info) {
+  MDSM_REQUIRE(tm && dm && snr && score && n > 0 && num_dms > 2 && dm_step > 0, MDSM_ERR_ARG,
+               "bad arguments");
+  std::vector<float> hist(num_dms, 0.f), smooth(num_dms, 0.f), model(num_dms, 0.f);
+  for (uint32_t i = 0; i < n; ++i) {
+    const int b = (int)std::round((dm[i] - min_dm) / dm_step);
+    MDSM_REQUIRE(b >= 0 && (uint32_t)b < num_dms, MDSM_ERR_ARG, "point %u outside the DM range", i);
+    hist[b] = (float)(hist[b] + snr[i]);
+  }
+  for (uint32_t i = 1; i + 1 < num_dms; ++i) smooth[i] = (float)((hist[i] + hist[i] + hist[i + 1]) / 3.0);
+  for (uint32_t i = 1; i + 1 < num_dms; ++i) hist[i] = smooth[i];
+  float peak = 0.f;
+  for (float v : hist) peak = std::max(peak, v);
+  const float inv_peak = (float)(1.0 / peak);
+  std::vector<int> high;
+  float level = 0.f;
+  for (uint32_t i = 0; i < num_dms; ++i)
+    if (hist[i] * inv_peak > 0.9) {
+      high.push_back((int)i);
+      level += hist[i];
+    }
+  const size_t h = high.size();
+  level /= (float)h;
+  const float max_dm = (h % 2 == 0) ? (float)((high[h / 2 - 1] + high[h / 2]) * 0.5 * dm_step + min_dm)
+                                    : (float)high[h / 2] * dm_step + min_dm;
+  float max_snr = 0.f, width = 0.f;
+  double position = 0.0;
+  float result = 1.f;
+  do {
+    if (max_dm <= 2.0) break;  // zero-DM RFI
+    for (uint32_t i = 0; i < n; ++i)
+      if (std::fabs(dm[i] - max_dm) < dm_step * 0.5 && snr[i] > max_snr) max_snr = (float)snr[i];
+    double t_lo = FLT_MAX, t_hi = FLT_MIN;
+    for (uint32_t i = 0; i < n; ++i)
+      if (std::fabs(dm[i] - max_dm) < dm_step) {
+        t_lo = std::min(t_lo, tm[i]);
+        t_hi = std::max(t_hi, tm[i]);
+      }
+    width = (float)((t_hi - t_lo) * 1e3 * 0.4);
+    position = t_lo + (t_hi - t_lo) / 2;
+    if (!(width <= FLT_MAX && width >= FLT_MIN && width > min_pulse_width)) break;
+    const float band = std::fabs(foff * (float)nchans);
+    const float freq3 = (float)std::pow(1e-3 * fch1 - (band * 1e-3) * 0.5, 3);
+    float fit_max = 0.f;
+    for (uint32_t i = 0; i < num_dms; ++i) {
+      const float x = (float)i * dm_step + min_dm - max_dm;
+      const float y = (float)(6.91e-3 * x * (band / (width * freq3)));
+      model[i] = (float)(std::sqrt(M_PI) * 0.5 * (1.0 / y) * std::erf(y));
+      fit_max = (fit_max > model[i]) ? fit_max : model[i];
+    }
+    model[(int)std::round((max_dm - min_dm) / dm_step)] = 1.f;
+    if ((fit_max - 1.0) > 0.1) break;
+    float mse = 0.f;
+    for (uint32_t i = 0; i < num_dms; ++i) {
+      const float d = hist[i] / level - model[i];
+      mse += d * d;
+    }
+    result = mse / (float)num_dms;
+  } while (false);
+  *score = result;
+  if (info) {
+    info[0] = max_dm;
+    info[1] = max_snr;
+    info[2] = width;
+    info[3] = position;
+  }
   return MDSM_OK;
 }

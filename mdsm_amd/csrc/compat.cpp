@@ -143,12 +143,33 @@ bool detect_beam(unsigned b, const float *d_x, unsigned nsamp_valid, double time
     mdsm_dbscan_ref(bs.ctime.data(), bs.cdm.data(), bs.csnr.data(), n,
                     (float)(s->dbscan_time_range * blockRate), s->dbscan_dm_range,
                     s->dbscan_snr_range, s->dbscan_min_points, bs.ccl.data(), &nc);
-    // clusters in id order, then noise (multibeam_dedispersion_output.cpp:262-312)
-    for (uint32_t id = 1; id <= nc; ++id)
+    // clusters in id order, then noise (multibeam_dedispersion_output.cpp:262-312);
+    // with classification a cluster scoring >= 0.1 is written with id -2
+    std::vector<double> ct, cd, cs;
+    for (uint32_t id = 1; id <= nc; ++id) {
+      int written_id = (int)id + (int)bs.clusters_written;
+      if (s->apply_classification) {
+        ct.clear();
+        cd.clear();
+        cs.clear();
+        for (uint32_t i = 0; i < n; ++i)
+          if (bs.ccl[i] == (int)id) {
+            ct.push_back(bs.ctime[i]);
+            cd.push_back(bs.cdm[i]);
+            cs.push_back(bs.csnr[i]);
+          }
+        float score = 1.f;
+        if (mdsm_classify_cluster_ref(ct.data(), cd.data(), cs.data(), (uint32_t)ct.size(), s->lowdm,
+                                      s->dmstep, s->tdms, s->min_pulse_width, s->beams[0].fch1,
+                                      s->beams[0].foff, s->nchans, &score, nullptr) != MDSM_OK)
+          score = 1.f;
+        if (!(score < 0.1)) written_id = -2;
+      }
       for (uint32_t i = 0; i < n; ++i)
         if (bs.ccl[i] == (int)id)
           fprintf(bs.fp, "%lf,%f,%f,%d,%d\n", bs.ctime[i], (float)bs.cdm[i], (float)bs.csnr[i],
-                  bs.ccl[i] + (int)bs.clusters_written, 0);
+                  written_id, 0);
+    }
     for (uint32_t i = 0; i < n; ++i)
       if (bs.ccl[i] == -1)
         fprintf(bs.fp, "%lf,%f,%f,%d,%d\n", bs.ctime[i], (float)bs.cdm[i], (float)bs.csnr[i], -1, 0);

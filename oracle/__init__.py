@@ -100,6 +100,7 @@ _sig("oracle_voltage_to_power", None, _f32p, _i64, _i, _i64, _i64)
 _sig("oracle_cnn_param_count", _i64, _i, _i)
 _sig("oracle_cnn_loss_grad", _d, _f64p, _i32p, _i, _i, _i, _f64p, C.c_void_p, C.c_void_p, _i)
 _sig("oracle_bf16_round", _d, _d)
+_sig("oracle_classify_ref", _f, _f64p, _f64p, _f64p, _i, _f, _f, _u32, _f, _f, _f, _u32, _f64p)
 _sig("oracle_rfi_inject", None, C.c_void_p, _i, _f32p, _u64, _f32p, _i64, _i64)
 _sig("oracle_snippet", None, _f32p, _i, _i, _f32p, _i64, _i, _i32p, _i, _i, _i, _i, _i, _f, _f, _f, _f32p)
 _sig("oracle_corner_turn", None, _f32p, _i64, _i, _i, _f32p, _i64, _i64, _i64)
@@ -588,4 +589,13 @@ def snippet(out, store, table, dm0, tc, SD=64, ST=256, SB=64, scale0=1.0, scale1
     _lib.oracle_snippet(out, out.shape[0], out.shape[1], store, store.shape[1], store.shape[0], table,
                         dm0, tc, SD, ST, SB, scale0, scale1, clip, res)
     return res
+
+
+def classify_ref(time, dm, snr, min_dm, dm_step, num_dms, min_pulse_width, fch1, foff, nchans):
+    """Reference cluster classification: (score, [maxDM, maxSNR, width, position])."""
+    info = np.zeros(4, np.float64)
+    t, d, s = (_c(a, np.float64) for a in (time, dm, snr))
+    score = _lib.oracle_classify_ref(t, d, s, t.size, min_dm, dm_step, num_dms, min_pulse_width, fch1, foff,
+                                     nchans, info)
+    return float(score), info
 

@@ -22,6 +22,7 @@
  *   brute-force sum             src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:44-95
  *   global mean/rms + threshold src/mdsm-multibeam/multibeam_dedispersion_output.cpp:148-185,188-248
  *   DBSCAN (3-D box variant)    src/mdsm-multibeam/dbscan.cpp:194-276,390-415
+ *   cluster classification      src/mdsm-multibeam/dbscan.cpp:25-171
  *   median of five (legacy)     src/mdsm/dedispersion_output.cpp:14-53,135-213
  *   detrend + normalise         src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:207-295
  *   SIGPROC sample unpack       src/mdsm-multibeam/file_handler.cpp:193-287
@@ -47,6 +48,11 @@
  *   dbscan2d           scikit-learn DBSCAN(metric="chebyshev")
  *   dbscan_ref         hand-derived cases from the reference algorithm's
  *                      text (the reference needs Qt headers: unbuildable here)
+ *   classify_ref       hand-derived cases (RFI and width rules, the
+ *                      histogram smoothing and median, the model curve's
+ *                      closed form at its peak); parity with the reference
+ *                      source is unpinned beyond them (it needs PGPLOT and
+ *                      Qt headers)
  *   median5            sorted median over random and exhaustive-order inputs
  *   f16 conversion     numpy float16 (round to nearest even)
  *   bandpass / RFI     exact sums on integer data, float64 means within
@@ -64,6 +70,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#define FLT_MAX_D 3.40282346638528859812e+38
+#define FLT_MIN_D 1.17549435082228750797e-38
+#define FLT_MAX_F 3.40282346638528859812e+38F
+#define FLT_MIN_F 1.17549435082228750797e-38F
+#define M_PI_D 3.14159265358979323846
 #define KDM_NORTHSTAR 4148.808
 #define KDM_REFERENCE 4148.741601
 #define R256 256
@@ -971,5 +982,87 @@ void oracle_voltage_to_power(float *x, int64_t ld, int nchans, int64_t col0, int
       memcpy(v, x + c * ld + col0 + s, 4);
       x[c * ld + col0 + s] = (float)(v[0] * v[0] + v[1] * v[1]);
     }
+}
+
+/* Cluster classification (Cluster::computeTransientProbability,
+ * dbscan.cpp:25-171) over the n member points of one cluster.  The float /
+ * double mix of the reference is kept: a float DM-S/N histogram, a 3-tap
+ * smoothing that reads (h[i] + h[i] + h[i+1]) / 3, the median DM of the
+ * bins above 0.9 of the maximum, the width from the time extent of points
+ * within one DM step (initial extremes FLT_MAX / FLT_MIN), and the mean
+ * squared difference from the model sqrt(pi)/2 erf(y)/y.  Returns 1 for a
+ * cluster rejected by a rule (DM <= 2, bad width, model out of range),
+ * else the MSE.  info = {maxDM, maxSNR, width, position}. */
+float oracle_classify_ref(const double *tm, const double *dm, const double *snr, int n, float minDm,
+                          float dmStep, unsigned numDMs, float min_pulse_width, float fch1,
+                          float foff, unsigned nchans, double *info) {
+  float *h = (float *)calloc(numDMs, sizeof(float));
+  float *tmp = (float *)calloc(numDMs, sizeof(float));
+  float *fit = (float *)calloc(numDMs, sizeof(float));
+  int *lst = (int *)malloc(sizeof(int) * (numDMs ? numDMs : 1));
+  float result = 1.0f, maxDM = 0, maxSNR = 0, width = 0;
+  double position = 0;
+  for (int i = 0; i < n; ++i) h[(int)round((dm[i] - minDm) / dmStep)] = (float)(h[(int)round((dm[i] - minDm) / dmStep)] + snr[i]);
+  for (unsigned i = 1; i + 1 < numDMs; ++i) tmp[i] = (float)((h[i] + h[i] + h[i + 1]) / 3.0);
+  for (unsigned i = 1; i + 1 < numDMs; ++i) h[i] = tmp[i];
+  float mx = 0;
+  for (unsigned i = 0; i < numDMs; ++i) mx = h[i] > mx ? h[i] : mx;
+  const float inv = (float)(1.0 / mx);
+  int size = 0;
+  mx = 0;
+  for (unsigned i = 0; i < numDMs; ++i)
+    if (h[i] * inv > 0.9) {
+      lst[size++] = (int)i;
+      mx += h[i];
+    }
+  mx /= (float)size;
+  if (size % 2 == 0)
+    maxDM = (float)((lst[size / 2 - 1] + lst[size / 2]) * 0.5 * dmStep + minDm);
+  else
+    maxDM = (float)lst[size / 2] * dmStep + minDm;
+  if (maxDM <= 2.0) goto done;
+  for (int i = 0; i < n; ++i)
+    if (fabs(dm[i] - maxDM) < dmStep * 0.5 && snr[i] > maxSNR) maxSNR = (float)snr[i];
+  {
+    double minT = FLT_MAX_D, maxT = FLT_MIN_D;
+    for (int i = 0; i < n; ++i)
+      if (fabs(dm[i] - maxDM) < dmStep) {
+        minT = tm[i] < minT ? tm[i] : minT;
+        maxT = tm[i] > maxT ? tm[i] : maxT;
+      }
+    width = (float)((maxT - minT) * 1e3 * 0.4);
+    position = minT + (maxT - minT) / 2;
+  }
+  if (!(width <= FLT_MAX_F && width >= FLT_MIN_F && width > min_pulse_width)) goto done;
+  {
+    const float band = fabsf(foff * (float)nchans);
+    const float freq = (float)pow(1e-3 * fch1 - (band * 1e-3) * 0.5, 3);
+    float fitMax = 0;
+    for (unsigned i = 0; i < numDMs; ++i) {
+      const float x = (float)i * dmStep + minDm - maxDM;
+      const float y = (float)(6.91e-3 * x * (band / (width * freq)));
+      fit[i] = (float)(sqrt(M_PI_D) * 0.5 * (1.0 / y) * erff(y));
+      fitMax = fitMax > fit[i] ? fitMax : fit[i];
+    }
+    fit[(int)roundf((maxDM - minDm) / dmStep)] = 1;
+    if ((fitMax - 1.0) > 0.1) goto done;
+    for (unsigned i = 0; i < numDMs; ++i) h[i] /= mx;
+    float mse = 0;
+    for (unsigned i = 0; i < numDMs; ++i) mse += (h[i] - fit[i]) * (h[i] - fit[i]);
+    mse /= (float)numDMs;
+    result = mse;
+  }
+done:
+  if (info) {
+    info[0] = maxDM;
+    info[1] = maxSNR;
+    info[2] = width;
+    info[3] = position;
+  }
+  free(h);
+  free(tmp);
+  free(fit);
+  free(lst);
+  return result;
 }
 

@@ -166,3 +166,61 @@ def test_streaming_manager_rfi_median_detrend(tmp_path):
         want = O.detrend_normalise(O.median_filter5(series), N).reshape(M, N)
         # replacement values come from two least-squares solvers (1 ulp at most)
         np.testing.assert_allclose(arr, want, rtol=1e-4, atol=1e-4, err_msg=f"chunk {k}")
+
+
+XML_CLASSIFY = """<observation>
+  <dm lowDM="0" numDMs="64" dmStep="1.0" />
+  <channels nchans="64" npols="1" />
+  <timing tsamp="0.000064" />
+  <samples nsamp="8192" nbits="32" />
+  <detection detectionThreshold="5" />
+  <clustering applyClustering="1" minPoints="3" timeRange="8" dmRange="3" snrRange="100"
+              applyClassification="1" minPulseWidth="0.1" />
+  <writer filePrefix="cls" baseDirectory="{dir}" singleFileMode="1" />
+  <beams><beam beamId="0" topFrequency="1500" frequencyOffset="-1.0" /></beams>
+</observation>"""
+
+
+def test_streaming_manager_classification(tmp_path):
+    xml = tmp_path / "obs.xml"
+    xml.write_text(XML_CLASSIFY.format(dir=str(tmp_path)))
+    assert lib.mdsm_compat_open(str(xml).encode(), 0) == 0, lib.mdsm_compat_last_error()
+    nb, nch, ns, nd, ms = (C.c_uint() for _ in range(5))
+    lib.mdsm_compat_survey(C.byref(nb), C.byref(nch), C.byref(ns), C.byref(nd), C.byref(ms))
+    C_, N, M, S = nch.value, ns.value, nd.value, ms.value
+    print(f"[compat classify] C={C_} N={N} M={M} maxshift={S}", file=sys.stderr, flush=True)
+    stream = O.generate(9, C_, 1500.0, -1.0, 64e-6, 0, S + N,
+                        pulses=[(40.0, S + 2000, 64, 1.2), (0.0, S + 6000, 4, 6.0)])
+    for i, n in enumerate([S, N, 0, 0]):
+        buf = lib.mdsm_compat_buffer()
+        if n:
+            np.ctypeslib.as_array(buf, shape=(C_, n))[:] = stream[:, :S] if i == 0 else stream[:, S:S + N]
+        samples = C.c_uint()
+        lib.mdsm_compat_next_chunk(n, C.byref(samples), 0.0, 64e-6)
+        if i == 1:
+            # the chunk was processed by start_processing below; grab its
+            # candidates after that call
+            pass
+        r = lib.mdsm_compat_start(n)
+        if i == 1:
+            t, d, s, cl = C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.POINTER(C.c_int)()
+            lib.mdsm_compat_candidates.restype = C.c_uint
+            cnt = lib.mdsm_compat_candidates(0, C.byref(t), C.byref(d), C.byref(s), C.byref(cl))
+            tm = np.ctypeslib.as_array(t, (cnt,)).copy()
+            dm = np.ctypeslib.as_array(d, (cnt,)).copy()
+            sn = np.ctypeslib.as_array(s, (cnt,)).copy()
+            lab = np.ctypeslib.as_array(cl, (cnt,)).copy()
+        if r == 0:
+            break
+    lib.mdsm_compat_close()
+    assert lab.max() >= 2, "expected at least two clusters"
+    expect = {}
+    for cid in range(1, lab.max() + 1):
+        m = lab == cid
+        score, _ = O.classify_ref(tm[m], dm[m], sn[m], 0.0, 1.0, M, 0.1, 1500.0, -1.0, C_)
+        expect[cid] = cid if score < 0.1 else -2
+    rows = [ln.split(",") for ln in (tmp_path / "cls.dat").read_text().strip().splitlines()]
+    written = [int(r[3]) for r in rows if int(r[3]) != -1]
+    want = [expect[cid] for cid in range(1, lab.max() + 1) for _ in range(int((lab == cid).sum()))]
+    assert written == want
+    assert -2 in want  # the zero-DM spike is rejected
