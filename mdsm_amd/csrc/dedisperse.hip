@@ -55,24 +55,26 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 
 // LDS byte offsets of the wave's trials inside the windows of one channel
 // block, relative to the block buffer plus the lane's own word.  Lane l of
-// register n holds the offset of (r, i) = divmod(64 n + l, MW), i.e. channel
-// cb+r and trial dw+i (both clamped into range), from rel = D[c][d] - D[c][d0]:
+// register n holds the offset of (r, i) = divmod(16 n + l % 16, MW) (the 16
+// values repeat in each row of 16 lanes), i.e. channel cb+r and trial dw+i
+// (both clamped into range), from rel = D[c][d] - D[c][d0]:
 //   PACK 0 (one sample per word):  word = r*rw + rel
 //   PACK 1 (pairs, window starts at element D[c][d0] of an even word
 //           origin): q = rel + (D[c][d0] & 1), word = r*rw + q/2 + (q&1)*rw/2
 //   PACK 2 (pairs, origin on a 4-sample boundary): as 1 with D[c][d0] & 3
 // (copy P is the first half of a row, copy Q the second).  Computed once per
-// block, so a trial costs one v_readlane and one address add.
+// block; a trial's address is then one v_add_u32 whose offset operand is
+// broadcast from lane i of each row by DPP row_newbcast (no v_readlane).
 template <int MW, int CB, int PACK>
 struct RelOffsets {
-  static constexpr int N = (CB * MW + 63) / 64;
+  static constexpr int N = (CB * MW + 15) / 16;
   int32_t v[N];
   __device__ __forceinline__ void load(const int32_t *__restrict__ table, uint32_t ndms,
                                        uint32_t nchans, uint32_t cb, uint32_t d0, uint32_t dw,
                                        uint32_t lane, uint32_t rw) {
 #pragma unroll
     for (int n = 0; n < N; ++n) {
-      const uint32_t idx = 64u * n + lane;
+      const uint32_t idx = 16u * n + (lane & 15u);
       const uint32_t r = idx / MW;
       const uint32_t c = min(cb + r, nchans - 1u);
       const uint32_t d = min(dw + idx % MW, ndms - 1u);
@@ -89,9 +91,19 @@ struct RelOffsets {
       v[n] = (int32_t)(4u * w);
     }
   }
+  // offset of (r, i) in every lane: DPP row_newbcast of lane idx % 16 (the
+  // callers' loops are unrolled, so the switch folds to one case)
   __device__ __forceinline__ int32_t get(int r, int i) const {
     const int idx = r * MW + i;
-    return __builtin_amdgcn_readlane(v[idx >> 6], idx & 63);
+    const int32_t x = v[idx >> 4];
+#define MDSM_BCAST(j) \
+  case j: return __builtin_amdgcn_update_dpp(0, x, 0x150 + j, 0xF, 0xF, false);
+    switch (idx & 15) {
+      MDSM_BCAST(0) MDSM_BCAST(1) MDSM_BCAST(2) MDSM_BCAST(3) MDSM_BCAST(4) MDSM_BCAST(5)
+      MDSM_BCAST(6) MDSM_BCAST(7) MDSM_BCAST(8) MDSM_BCAST(9) MDSM_BCAST(10) MDSM_BCAST(11)
+      MDSM_BCAST(12) MDSM_BCAST(13) MDSM_BCAST(14) default: return __builtin_amdgcn_update_dpp(0, x, 0x15F, 0xF, 0xF, false);
+    }
+#undef MDSM_BCAST
   }
 };
 
