@@ -484,6 +484,39 @@ int mdsm_sigproc_close(mdsm_sigproc_reader reader);
 int mdsm_sigproc_read(mdsm_sigproc_reader reader, uint32_t nspectra, float *d_out, uint64_t ld,
                       uint64_t beam_stride, uint64_t col0, uint32_t *got, void *stream);
 
+/* Per-DM SIGPROC output (DedispersedDataWriter,
+ * src/pelican-mdsm/modules/DedispersedDataWriter.cpp:17-135): one file
+ * "<prefix>_<dm %g>.fil" per requested DM with the reference's header
+ * (machine_id 0, telescope_id 0, data_type 1, fch1, foff, nchans 1, tsamp,
+ * nbits 32, tstart, nifs 1); each write appends, for every requested DM,
+ * the FP32 host row of `series` [nrows][nsamp] whose trial DM
+ * lowdm + j*dmstep is within 1e-5 of it. */
+typedef struct mdsm_dm_writer_s *mdsm_dm_writer;
+int mdsm_dm_writer_open(const char *prefix, const float *dms, uint32_t ndms, double fch1, double foff,
+                        double tsamp, double tstart, mdsm_dm_writer *writer);
+int mdsm_dm_writer_write(mdsm_dm_writer writer, const float *series, uint32_t nrows, uint32_t nsamp,
+                         float lowdm, float dmstep);
+int mdsm_dm_writer_close(mdsm_dm_writer writer);
+
+/* Logarithmic ("mu-law") quantisation of raw beam data for the disk writer
+ * (quantise_power_data / quantise_complex_data,
+ * multibeam_dedispersion_writer.cpp:21-118).  Power: byte =
+ * ((int)(d/Q)*Q + Q/2) * 2^bits with d = log10(1 + k*mu/32768)/log10(1+mu),
+ * k = (int)(x / max * 32768), max = twice the maximum of the first buffer
+ * (state->max_value 0 on the first call; kept afterwards).  Complex (int16
+ * re, im pairs): 3-bit magnitudes + sign bits, [rs r r r is i i i].  The
+ * reference overruns its table for values above max and for -32768; these
+ * are clamped to the last entry. */
+typedef struct {
+  uint32_t bits;         /* output bits (outputBits), 1..8 */
+  uint32_t compression;  /* mu (compression) */
+  float max_value;       /* 0 before the first buffer */
+} mdsm_mulaw_state_t;
+int mdsm_mulaw_encode_power(const float *d_in, uint64_t n, mdsm_mulaw_state_t *state, uint8_t *d_out,
+                            void *stream);
+int mdsm_mulaw_encode_complex(const int16_t *d_in, uint64_t n, uint32_t bits, uint32_t compression,
+                              uint8_t *d_out, void *stream);
+
 /* ------------------------------------------------------- synthetic input
  * Seeded synthetic filterbank data generated on the device (Philox4x32-10
  * counters, Box-Muller normals): raw[c][t0+i] = gain[c]*(z + pulses) +

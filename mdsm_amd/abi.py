@@ -539,6 +539,29 @@ _sig("mdsm_unpack_corner_turn", _vp, _i, _u32, _u32, _u32, _vp, _u64, _u64, _u64
 _sig("mdsm_sigproc_open", C.c_char_p, _u32, _u32, _u32, _P(_vp), _P(SigprocHeader))
 _sig("mdsm_sigproc_close", _vp)
 _sig("mdsm_sigproc_read", _vp, _u32, _vp, _u64, _u64, _u64, _P(_u32), _vp)
+_sig("mdsm_dm_writer_open", C.c_char_p, _vp, _u32, _d, _d, _d, _d, _P(_vp))
+_sig("mdsm_dm_writer_write", _vp, _vp, _u32, _u32, _f, _f)
+_sig("mdsm_dm_writer_close", _vp)
+
+
+class DmWriter:
+    """Per-DM SIGPROC files of dedispersed series (host rows)."""
+
+    def __init__(self, prefix, dms, fch1, foff, tsamp, tstart=0.0):
+        self.dms = np.ascontiguousarray(dms, np.float32)
+        self.handle = _vp()
+        check(_lib.mdsm_dm_writer_open(prefix.encode(), _ptr(self.dms), self.dms.size, fch1, foff, tsamp,
+                                       tstart, C.byref(self.handle)))
+
+    def write(self, series, lowdm, dmstep):
+        series = np.ascontiguousarray(series, np.float32)
+        check(_lib.mdsm_dm_writer_write(self.handle, _ptr(series), series.shape[0], series.shape[1], lowdm,
+                                        dmstep))
+
+    def close(self):
+        if self.handle:
+            check(_lib.mdsm_dm_writer_close(self.handle))
+            self.handle = None
 
 
 def sigproc_read_header(path: str) -> SigprocHeader:

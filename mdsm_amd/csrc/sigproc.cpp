@@ -17,9 +17,11 @@
 // tries it as a keyword); unset fields are zero.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "internal.h"
 
@@ -289,5 +291,91 @@ extern "C" int mdsm_sigproc_read(mdsm_sigproc_reader r, uint32_t nspectra, float
     r->slot ^= 1;
   }
   *got = n;
+  return MDSM_OK;
+}
+
+// ---------------------------------------------------------------- writer
+// Per-DM SIGPROC output (DedispersedDataWriter,
+// src/pelican-mdsm/modules/DedispersedDataWriter.cpp:17-135): one file
+// <prefix>_<dm>.fil per requested DM (the DM printed as %g), a header of
+// machine_id 0, telescope_id 0, data_type 1, fch1, foff, nchans 1, tsamp,
+// nbits 32, tstart, nifs 1 in that order, then the FP32 samples of the
+// trial whose DM is within 1e-5 of the requested one, appended per chunk.
+struct mdsm_dm_writer_s {
+  std::vector<FILE *> files;
+  std::vector<float> dms;
+};
+
+namespace {
+void put_string(FILE *f, const char *s) {
+  const int32_t n = (int32_t)std::strlen(s);
+  fwrite(&n, 4, 1, f);
+  fwrite(s, 1, (size_t)n, f);
+}
+void put_int(FILE *f, const char *k, int32_t v) {
+  put_string(f, k);
+  fwrite(&v, 4, 1, f);
+}
+void put_double(FILE *f, const char *k, double v) {
+  put_string(f, k);
+  fwrite(&v, 8, 1, f);
+}
+}  // namespace
+
+extern "C" int mdsm_dm_writer_open(const char *prefix, const float *dms, uint32_t ndms, double fch1,
+                                   double foff, double tsamp, double tstart, mdsm_dm_writer *out) {
+  MDSM_REQUIRE(prefix && dms && ndms && out, MDSM_ERR_ARG, "null argument");
+  *out = nullptr;
+  auto *w = new mdsm_dm_writer_s();
+  for (uint32_t i = 0; i < ndms; ++i) {
+    char path[4096];
+    snprintf(path, sizeof(path), "%s_%g.fil", prefix, (double)dms[i]);
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+      for (FILE *g : w->files) fclose(g);
+      delete w;
+      set_error("cannot create %s", path);
+      return MDSM_ERR_IO;
+    }
+    put_string(f, "HEADER_START");
+    put_int(f, "machine_id", 0);
+    put_int(f, "telescope_id", 0);
+    put_int(f, "data_type", 1);
+    put_double(f, "fch1", fch1);
+    put_double(f, "foff", foff);
+    put_int(f, "nchans", 1);
+    put_double(f, "tsamp", tsamp);
+    put_int(f, "nbits", 32);
+    put_double(f, "tstart", tstart);
+    put_int(f, "nifs", 1);
+    put_string(f, "HEADER_END");
+    fflush(f);
+    w->files.push_back(f);
+    w->dms.push_back(dms[i]);
+  }
+  *out = w;
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_dm_writer_write(mdsm_dm_writer w, const float *series, uint32_t nrows,
+                                    uint32_t nsamp, float lowdm, float dmstep) {
+  MDSM_REQUIRE(w && (series || !nrows), MDSM_ERR_ARG, "null argument");
+  for (size_t i = 0; i < w->files.size(); ++i)
+    for (uint32_t j = 0; j < nrows; ++j) {
+      const float dm = lowdm + dmstep * (float)j;
+      if (std::fabs(dm - w->dms[i]) < 0.00001) {
+        MDSM_REQUIRE(fwrite(series + (uint64_t)j * nsamp, 4, nsamp, w->files[i]) == nsamp, MDSM_ERR_IO,
+                     "write failed");
+        fflush(w->files[i]);
+        break;
+      }
+    }
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_dm_writer_close(mdsm_dm_writer w) {
+  if (!w) return MDSM_OK;
+  for (FILE *f : w->files) fclose(f);
+  delete w;
   return MDSM_OK;
 }

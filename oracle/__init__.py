@@ -100,6 +100,8 @@ _sig("oracle_voltage_to_power", None, _f32p, _i64, _i, _i64, _i64)
 _sig("oracle_cnn_param_count", _i64, _i, _i)
 _sig("oracle_cnn_loss_grad", _d, _f64p, _i32p, _i, _i, _i, _f64p, C.c_void_p, C.c_void_p, _i)
 _sig("oracle_bf16_round", _d, _d)
+_sig("oracle_mulaw_power", None, _f32p, _i64, _u32, _u32, _f, _u8p)
+_sig("oracle_mulaw_complex", None, np.ctypeslib.ndpointer(np.int16, flags="C_CONTIGUOUS"), _i64, _u32, _u32, _u8p)
 _sig("oracle_classify_ref", _f, _f64p, _f64p, _f64p, _i, _f, _f, _u32, _f, _f, _f, _u32, _f64p)
 _sig("oracle_rfi_inject", None, C.c_void_p, _i, _f32p, _u64, _f32p, _i64, _i64)
 _sig("oracle_snippet", None, _f32p, _i, _i, _f32p, _i64, _i, _i32p, _i, _i, _i, _i, _i, _f, _f, _f, _f32p)
@@ -598,4 +600,18 @@ def classify_ref(time, dm, snr, min_dm, dm_step, num_dms, min_pulse_width, fch1,
     score = _lib.oracle_classify_ref(t, d, s, t.size, min_dm, dm_step, num_dms, min_pulse_width, fch1, foff,
                                      nchans, info)
     return float(score), info
+
+
+def mulaw_power(x, bits, mu, max_value):
+    x = _c(x, np.float32).reshape(-1)
+    out = np.empty(x.size, np.uint8)
+    _lib.oracle_mulaw_power(x, x.size, bits, mu, max_value, out)
+    return out
+
+
+def mulaw_complex(pairs, bits, mu):
+    p = _c(pairs, np.int16).reshape(-1)
+    out = np.empty(p.size // 2, np.uint8)
+    _lib.oracle_mulaw_complex(p, out.size, bits, mu, out)
+    return out
 

@@ -23,6 +23,7 @@
  *   global mean/rms + threshold src/mdsm-multibeam/multibeam_dedispersion_output.cpp:148-185,188-248
  *   DBSCAN (3-D box variant)    src/mdsm-multibeam/dbscan.cpp:194-276,390-415
  *   cluster classification      src/mdsm-multibeam/dbscan.cpp:25-171
+ *   mu-law raw-data quantiser   src/mdsm-multibeam/multibeam_dedispersion_writer.cpp:21-118
  *   median of five (legacy)     src/mdsm/dedispersion_output.cpp:14-53,135-213
  *   detrend + normalise         src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:207-295
  *   SIGPROC sample unpack       src/mdsm-multibeam/file_handler.cpp:193-287
@@ -60,6 +61,8 @@
  *                      flagged blocks and clipped spectra are known; the
  *                      least-squares fit itself is numpy's SVD lstsq (the
  *                      method of the reference's gsl_multifit_linear)
+ *   mu-law quantiser   closed forms (0 -> Q/2 level, the table's end -> top
+ *                      level), monotonicity, sign bits of complex pairs
  *   sigproc unpack     the reference's read_block compiled from its own
  *                      source (oracle/_ref/libref_file.so) and the values
  *                      the fixture writer packed (tests/golden)
@@ -1064,5 +1067,52 @@ done:
   free(fit);
   free(lst);
   return result;
+}
+
+/* Mu-law quantiser (quantise_power_data / quantise_complex_data,
+ * multibeam_dedispersion_writer.cpp:21-118, LOG_LOOKUP_LENGTH 32768):
+ * table log10(1 + i*mu/32768) in float; power values scaled by 32768/max to
+ * a table index (clamped to the table; the reference reads past it),
+ * quantised as ((int)(d/Q)*Q + Q/2) * 2^bits and stored through int to a
+ * byte; complex pairs as [sign_re, 3 bits, sign_im, 3 bits]. */
+static void mulaw_table(unsigned mu, float *lut) {
+  for (unsigned i = 0; i < 32768; ++i) lut[i] = log10f(1.0f + (float)(i * mu) / 32768.0f);
+}
+
+void oracle_mulaw_power(const float *x, int64_t n, unsigned bits, unsigned mu, float max_value,
+                        uint8_t *out) {
+  float *lut = (float *)malloc(32768 * sizeof(float));
+  mulaw_table(mu, lut);
+  const float Q = (float)(1.0 / pow(2, bits)), invQ = (float)(1.0 / Q);
+  const float l1pmu = (float)(1.0 / log10(1 + mu)), inv_max = (float)(1.0 / max_value);
+  const unsigned bitrange = (unsigned)pow(2, bits);
+  for (int64_t i = 0; i < n; ++i) {
+    float d = x[i] * inv_max * 32768;
+    int k = (int)d;
+    k = k < 0 ? 0 : (k > 32767 ? 32767 : k);
+    d = lut[k] * l1pmu;
+    out[i] = (uint8_t)(int)(((float)(int)(d * invQ) * Q + Q * 0.5) * bitrange);
+  }
+  free(lut);
+}
+
+void oracle_mulaw_complex(const int16_t *x, int64_t n, unsigned bits, unsigned mu, uint8_t *out) {
+  float *lut = (float *)malloc(32768 * sizeof(float));
+  mulaw_table(mu, lut);
+  const unsigned bitrange = (unsigned)pow(2, bits);
+  const float Q = (float)(1.0 / bitrange), invQ = (float)(1.0 / Q);
+  const float l1pmu = (float)(1.0 / log10(1 + mu));
+  for (int64_t i = 0; i < n; ++i) {
+    const int re = x[2 * i], im = x[2 * i + 1];
+    int ar = abs(re), ai = abs(im);
+    ar = ar > 32767 ? 32767 : ar;
+    ai = ai > 32767 ? 32767 : ai;
+    const float dr = lut[ar] * l1pmu, di = lut[ai] * l1pmu;
+    const uint8_t qr = (uint8_t)(int)(((float)(signed char)(dr * invQ) * Q + Q * 0.5) * bitrange);
+    const uint8_t qi = (uint8_t)(int)(((float)(signed char)(di * invQ) * Q + Q * 0.5) * bitrange);
+    const signed char rs = re < 0 ? -1 : 1, is = im < 0 ? -1 : 1;
+    out[i] = (uint8_t)((rs & 0x80) | ((qr & 0x07) << 4) | ((is & 0x80) >> 4) | (qi & 0x07));
+  }
+  free(lut);
 }
 

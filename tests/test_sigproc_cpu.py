@@ -148,3 +148,33 @@ def test_reader_open_errors():
     # 64 channels in the header cannot be 3 beams x 20
     assert lib.mdsm_sigproc_open(fil, 20, 3, 16, abi.C.byref(r), None) == abi.MDSM_ERR_ARG
     assert lib.mdsm_sigproc_open(fil, 0, 3, 16, abi.C.byref(r), None) == abi.MDSM_ERR_ARG
+
+
+def test_dm_writer_round_trip(tmp_path):
+    rng = np.random.default_rng(8)
+    series = rng.standard_normal((130, 300)).astype(np.float32)
+    prefix = str(tmp_path / "MDSM")
+    w = abi.DmWriter(prefix, [0.0, 12.3, 3.5, 99.0], fch1=150.0, foff=-0.1953125, tsamp=2.56e-4)
+    w.write(series[:, :200], lowdm=0.0, dmstep=0.1)
+    w.write(series[:, 200:], lowdm=0.0, dmstep=0.1)
+    w.close()
+    import struct
+    for dm, row in [(0.0, 0), (12.3, 123), (3.5, 35)]:
+        path = f"{prefix}_{dm:g}.fil"
+        h = abi.sigproc_read_header(path)
+        assert (h.nchans, h.nbits, h.nifs, h.data_type, h.machine_id) == (1, 32, 1, 1, 0)
+        assert (h.fch1, h.foff, h.tsamp, h.tstart) == (150.0, -0.1953125, 2.56e-4, 0.0)
+        data = np.fromfile(path, np.float32, offset=h.data_offset)
+        np.testing.assert_array_equal(data, series[row])
+        # the header bytes, keyword by keyword, in the reference's order
+        blob = open(path, "rb").read()[:h.data_offset]
+        want = mkfil._s(b"", "HEADER_START")
+        for k, fmt, v in [("machine_id", "<i", 0), ("telescope_id", "<i", 0), ("data_type", "<i", 1),
+                          ("fch1", "<d", 150.0), ("foff", "<d", -0.1953125), ("nchans", "<i", 1),
+                          ("tsamp", "<d", 2.56e-4), ("nbits", "<i", 32), ("tstart", "<d", 0.0),
+                          ("nifs", "<i", 1)]:
+            want = mkfil._s(want, k) + struct.pack(fmt, v)
+        assert blob == mkfil._s(want, "HEADER_END")
+    # a DM with no matching trial gets a header and no samples
+    h = abi.sigproc_read_header(f"{prefix}_99.fil")
+    assert os.path.getsize(f"{prefix}_99.fil") == h.data_offset
