@@ -33,7 +33,9 @@ sys.path.insert(0, ROOT)
 from mdsm_amd import abi  # noqa: E402
 
 HBM_PEAK = 8.0e12        # MI355X HBM3E, bytes/s
-VALU_ADD_PEAK = 78.6e12  # FP32 vector adds/s with packed FP32 (157.3 TFLOP/s FMA)
+VALU_ADD_PEAK = 39.3e12  # FP32 vector adds/s, one per lane per clock (157.3 TFLOP/s counts
+                         # packed FMA as 4 flops; packed FP32 is not used by this build)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch_fp16.json")
 FCH1, BAND, TSAMP = 1500.0, 300.0, 64e-6
 
 
@@ -318,6 +320,23 @@ def main():
     for d, _ in ins16:
         d.free()
     del batch16
+    if args.profile:
+        # calibration for the PMC byte counters: one device-to-device copy of 1 GiB
+        cal_src, cal_dst = abi.DeviceBuffer(1 << 30), abi.DeviceBuffer(1 << 30)
+        abi.check(abi.lib().mdsm_memcpy_dtod(cal_dst.ptr, cal_src.ptr, 1 << 30, stream))
+        abi.synchronize()
+        cal_src.free()
+        cal_dst.free()
+
+    u8 = None
+    if not args.profile:
+        batch8, ins8, ms8, B8, v8 = run_batch(abi.U8, "u8")
+        u8 = {"ms": ms8, "dm_samples_per_s": total_work / ms8 * 1e3, "tile_variant": v8,
+              "hbm_frac": B8 / (ms8 / 1e3) / HBM_PEAK,
+              "adds_per_s": total_adds / (ms8 / 1e3)}
+        for d, _ in ins8:
+            d.free()
+        del batch8
 
     plain = None
     if not args.no_plain:
@@ -351,6 +370,11 @@ def main():
 
     value = total_work / ms16 * 1e3
     achieved = B16 / (ms16 / 1e3)
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):  # measured HBM bytes per launch (rocprofv3 --pmc)
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        traffic = tr.get("bytes_per_launch")
     line = {
         "metric": "dedispersed DM x samples per second (batched, FP16 storage)",
         "value": value,
@@ -370,9 +394,12 @@ def main():
                    "beams": len(beams), "sum_dm_samples": total_work, "sum_adds": total_adds,
                    "tile_variant": v16},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None},
-        "valu": {"adds_per_s": total_adds / (ms16 / 1e3), "peak_packed_fp32": VALU_ADD_PEAK,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": B16},
+        "valu": {"adds_per_s": total_adds / (ms16 / 1e3), "peak_adds_per_s": VALU_ADD_PEAK,
                  "frac": total_adds / (ms16 / 1e3) / VALU_ADD_PEAK},
+        "batched_u8": u8,
+        "batched_u8_vs_plain_fp32": (u8["dm_samples_per_s"] / plain["dm_samples_per_s"]) if (u8 and plain) else None,
         "plain_fp32_per_beam": plain,
         "batched_fp16_vs_plain_fp32": (value / plain["dm_samples_per_s"]) if plain else None,
         "single_beam_fp32": result.get("single_fp32"),

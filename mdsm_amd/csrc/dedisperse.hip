@@ -446,7 +446,8 @@ __global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__
   while (true) {
     if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
     __syncthreads();
-    const uint32_t item = item_s;
+    // wave-uniform: the descriptor lookup and every tile parameter stay scalar
+    const uint32_t item = uniform(item_s);
     __syncthreads();
     if (item >= total_items) break;
     uint32_t lo = 0, hi = nbeams - 1u;

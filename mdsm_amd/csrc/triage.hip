@@ -35,12 +35,13 @@ __device__ __forceinline__ uint32_t philox_u32(uint32_t c0, uint32_t c1, uint32_
   return c[0];
 }
 
-// one block row per RFI item, threads over its samples
+// one launch per RFI item (items on one channel add to the same samples, so
+// they are applied in order, as the oracle does), threads over its samples
 __global__ __launch_bounds__(TNT) void rfi_inject_kernel(const RfiItem *__restrict__ items,
-                                                         uint32_t nitems, const float *__restrict__ gain,
+                                                         uint32_t index, const float *__restrict__ gain,
                                                          uint64_t seed, float *__restrict__ raw,
                                                          uint64_t ld, uint64_t n) {
-  const RfiItem it = items[blockIdx.y];
+  const RfiItem it = items[index];
   float *row = raw + (uint64_t)it.chan * ld;
   const float g = gain[it.chan];
   const int64_t end = it.start + it.len < (int64_t)n ? it.start + it.len : (int64_t)n;
@@ -133,9 +134,11 @@ int launch_rfi_inject(const RfiItem *d_items, uint32_t nitems, int64_t max_len, 
   if (nitems == 0) return MDSM_OK;
   uint64_t g = (uint64_t)(max_len + TNT - 1) / TNT;
   g = g < 1 ? 1 : (g > 1024 ? 1024 : g);
-  hipLaunchKernelGGL(rfi_inject_kernel, dim3((unsigned)g, nitems), dim3(TNT), 0, s, d_items, nitems,
-                     gain, seed, raw, ld, n);
-  MDSM_HIP(hipGetLastError());
+  for (uint32_t i = 0; i < nitems; ++i) {
+    hipLaunchKernelGGL(rfi_inject_kernel, dim3((unsigned)g), dim3(TNT), 0, s, d_items, i, gain, seed,
+                       raw, ld, n);
+    MDSM_HIP(hipGetLastError());
+  }
   return MDSM_OK;
 }
 

@@ -19,7 +19,7 @@ def test_inject_rfi_matches_oracle():
     gain = rng.uniform(0.5, 2, C).astype(np.float32)
     items = [abi.RfiItem(3, abi.RFI_BURST, 100, 700, 12.5, 1.0),
              abi.RfiItem(7, abi.RFI_MAINS, 0, n, 6.0, 312.5),
-             abi.RfiItem(7, abi.RFI_TELEGRAPH, 50, 4000, 9.0, 64.0),
+             abi.RfiItem(7, abi.RFI_TELEGRAPH, 50, 4000, 9.0, 64.0),  # same channel: applied in order
              abi.RfiItem(15, abi.RFI_BURST, 4990, 100, 3.0, 1.0)]
     d_raw = abi.DeviceBuffer.from_numpy(raw)
     d_gain = abi.DeviceBuffer.from_numpy(gain)
