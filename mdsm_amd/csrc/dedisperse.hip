@@ -418,6 +418,15 @@ using F16Tile = PackedTile<MDSM_F16, MW, K2, CB>;
 template <int MW, int K2, int CB>
 using U8Tile = PackedTile<MDSM_U8, MW, K2, CB>;
 
+// XCD-aware workgroup order (on unless MDSM_XCD_REMAP=0 in the environment)
+bool xcd_remap_enabled() {
+  static const bool on = [] {
+    const char *e = getenv("MDSM_XCD_REMAP");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // ================================================================= kernels
 template <typename Tile>
 __global__ __launch_bounds__(256) void dedisp_kernel(const typename Tile::T *__restrict__ in,
@@ -425,12 +434,26 @@ __global__ __launch_bounds__(256) void dedisp_kernel(const typename Tile::T *__r
                                                      const int32_t *__restrict__ table,
                                                      uint32_t nchans, uint32_t nsamp,
                                                      uint32_t ndms, uint32_t rw, uint32_t cbr,
-                                                     uint32_t n_dtiles, float *__restrict__ out) {
+                                                     uint32_t n_dtiles, uint32_t n_ttiles,
+                                                     uint32_t xcd_remap, float *__restrict__ out) {
   extern __shared__ float lds[];
   // DM tiles vary fastest: the workgroups resident at one time then share a
-  // narrow time range of the input (reuse in L2 and the 256 MB MALL).
+  // narrow time range of the input.  Workgroup IDs are dealt round-robin over
+  // the 8 XCDs, each with its own L2; with xcd_remap, XCD x takes the time
+  // tiles tt = x (mod 8), so all DM tiles re-reading one time window run
+  // behind one L2 (grid: 8 * ceil(n_ttiles / 8) * n_dtiles).
   const uint32_t b = blockIdx.x;
-  Tile::run(in, ld, table, nchans, nsamp, ndms, rw, cbr, out, b / n_dtiles, b % n_dtiles, lds);
+  uint32_t tt, dt;
+  if (xcd_remap) {
+    const uint32_t x = b & 7u, k = b >> 3;
+    tt = ((k / n_dtiles) << 3) + x;
+    dt = k % n_dtiles;
+    if (tt >= n_ttiles) return;  // whole workgroup, before any barrier
+  } else {
+    tt = b / n_dtiles;
+    dt = b % n_dtiles;
+  }
+  Tile::run(in, ld, table, nchans, nsamp, ndms, rw, cbr, out, tt, dt, lds);
 }
 
 // Persistent multi-beam kernel: workgroups pull items from a queue until it
@@ -553,14 +576,16 @@ int launch_dedisperse(int dtype, int variant, const void *in, uint64_t ld, const
     using Tile = std::remove_pointer_t<decltype(tp)>;
     const uint32_t n_ttiles = (nsamp + Tile::TT - 1) / Tile::TT;
     const uint32_t n_dtiles = (ndms + Tile::TD - 1) / Tile::TD;
-    const uint64_t blocks = (uint64_t)n_ttiles * n_dtiles;
+    const uint32_t remap = xcd_remap_enabled() ? 1u : 0u;
+    const uint64_t blocks =
+        (uint64_t)(remap ? (n_ttiles + 7u) / 8u * 8u : n_ttiles) * n_dtiles;
     MDSM_REQUIRE(blocks < (1ull << 31), MDSM_ERR_ARG, "dedispersion grid too large");
     const uint32_t cbr = stage_channels<Tile>(rw);
     MDSM_REQUIRE(cbr >= 1, MDSM_ERR_ARG, "delay spread too large for LDS staging (%u words)", rw);
     const size_t lds = (size_t)cbr * Tile::lds_words_per_row(rw) * sizeof(float);
     hipLaunchKernelGGL(dedisp_kernel<Tile>, dim3((uint32_t)blocks), dim3(256), lds, s,
                        reinterpret_cast<const typename Tile::T *>(in), ld, table, nchans, nsamp,
-                       ndms, rw, cbr, n_dtiles, out);
+                       ndms, rw, cbr, n_dtiles, n_ttiles, remap, out);
     MDSM_HIP(hipGetLastError());
     return MDSM_OK;
   });
