@@ -235,6 +235,12 @@ int mdsm_batch_launch(mdsm_batch batch, void *stream);
 int mdsm_batch_dedisperse(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                           float *const *d_out, void *stream);
 int mdsm_batch_set_variant(mdsm_batch batch, int variant);
+/* Launch shape (takes effect at the next bind): PERSISTENT (default) runs
+ * resident workgroups that drain a device queue of tiles; FLAT launches one
+ * workgroup per tile, numbered so that the DM tiles re-reading one input
+ * window share an XCD (and its L2).  Both are one launch per step. */
+enum { MDSM_BATCH_PERSISTENT = 0, MDSM_BATCH_FLAT = 1 };
+int mdsm_batch_set_mode(mdsm_batch batch, int mode);
 
 /* ------------------------------------------------------- search pipeline
  * The per-chunk sequence of a multi-beam search: for every beam bandpass

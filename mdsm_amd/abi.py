@@ -117,6 +117,8 @@ _sig("mdsm_batch_bind", _vp, _P(_vp), _P(_u64), _P(_vp))
 _sig("mdsm_batch_launch", _vp, _vp)
 _sig("mdsm_batch_dedisperse", _vp, _P(_vp), _P(_u64), _P(_vp), _vp)
 _sig("mdsm_batch_set_variant", _vp, _i)
+_sig("mdsm_batch_set_mode", _vp, _i)
+BATCH_PERSISTENT, BATCH_FLAT = 0, 1
 _sig("mdsm_generate", C.c_uint64, _u32, C.c_int64, _u64, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp,
      _u64, _vp)
 
@@ -398,6 +400,9 @@ class Batch:
 
     def set_variant(self, v):
         check(_lib.mdsm_batch_set_variant(self.handle, v))
+
+    def set_mode(self, mode):
+        check(_lib.mdsm_batch_set_mode(self.handle, mode))
 
     def bind(self, d_in, lds, d_out):
         n = len(self.surveys)

@@ -21,6 +21,8 @@ using namespace mdsm;
 struct mdsm_batch_s {
   int dtype = MDSM_F32;
   int variant = 0;
+  int mode = MDSM_BATCH_PERSISTENT;
+  uint32_t bound_mode = MDSM_BATCH_PERSISTENT;
   std::vector<mdsm_survey_t> beams;
   std::vector<uint32_t> maxshift;
   std::vector<std::vector<int32_t>> h_tables;
@@ -169,22 +171,37 @@ extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const 
     d.n_dtiles = (s.ndms + sh.td() - 1) / sh.td();
     d.item_begin = (uint32_t)items;
     d.pad = 0;
-    items += (uint64_t)d.n_ttiles * d.n_dtiles;
+    // flat mode numbers items XCD-major: 8 slots per time-tile group
+    const uint64_t tgroups = batch->mode == MDSM_BATCH_FLAT ? (d.n_ttiles + 7u) / 8u * 8u : d.n_ttiles;
+    items += tgroups * d.n_dtiles;
     maxrw = std::max(maxrw, d.row_words);
   }
-  MDSM_REQUIRE(items < (1ull << 32) - 65536, MDSM_ERR_ARG, "too many work items");
+  MDSM_REQUIRE(items < (1ull << 31), MDSM_ERR_ARG, "too many work items");
   MDSM_HIP(hipMemcpy(batch->d_desc, batch->h_desc.data(), sizeof(BeamDesc) * nb,
                      hipMemcpyHostToDevice));
   batch->total_items = (uint32_t)items;
   batch->max_row_words = maxrw;
   batch->bound = true;
   batch->bound_variant = (uint32_t)v;
+  batch->bound_mode = (uint32_t)batch->mode;
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_batch_set_mode(mdsm_batch batch, int mode) {
+  MDSM_REQUIRE(batch && (mode == MDSM_BATCH_PERSISTENT || mode == MDSM_BATCH_FLAT), MDSM_ERR_ARG,
+               "mode must be MDSM_BATCH_PERSISTENT or MDSM_BATCH_FLAT");
+  batch->mode = mode;
+  batch->bound = false;
   return MDSM_OK;
 }
 
 extern "C" int mdsm_batch_launch(mdsm_batch batch, void *stream) {
   MDSM_REQUIRE(batch && batch->bound, MDSM_ERR_STATE, "batch not bound to buffers");
   hipStream_t s = as_stream(stream);
+  if (batch->bound_mode == MDSM_BATCH_FLAT)
+    return launch_batch_flat(batch->dtype, (int)batch->bound_variant, batch->d_desc,
+                             (uint32_t)batch->beams.size(), batch->total_items,
+                             batch->max_row_words, s);
   int rc = launch_zero_u32(batch->d_counter, 1, s);
   if (rc != MDSM_OK) return rc;
   return launch_batch_dedisperse(batch->dtype, (int)batch->bound_variant, batch->d_desc,

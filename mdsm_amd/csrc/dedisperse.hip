@@ -490,6 +490,32 @@ __global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__
   }
 }
 
+// Flat multi-beam kernel: one workgroup per item, all beams in one launch.
+// Items of a beam are numbered XCD-major as in dedisp_kernel (its item
+// count is 8 * ceil(n_ttiles / 8) * n_dtiles and item_begin is a multiple
+// of 8, so blockIdx % 8 is the item's XCD slot).
+template <typename Tile>
+__global__ __launch_bounds__(256) void batch_flat_kernel(const BeamDesc *__restrict__ beams,
+                                                         uint32_t nbeams, uint32_t cbr) {
+  extern __shared__ float lds[];
+  const uint32_t item = blockIdx.x;
+  uint32_t lo = 0, hi = nbeams - 1u;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    if (beams[mid].item_begin <= item) lo = mid;
+    else hi = mid - 1u;
+  }
+  const BeamDesc &bd = beams[lo];
+  const uint32_t local = item - bd.item_begin;
+  const uint32_t x = local & 7u, k = local >> 3;
+  const uint32_t tt = ((k / bd.n_dtiles) << 3) + x, dt = k % bd.n_dtiles;
+  if (tt >= bd.n_ttiles) return;  // whole workgroup, before any barrier
+  char *base = reinterpret_cast<char *>(const_cast<BeamDesc *>(beams));
+  Tile::run(reinterpret_cast<const typename Tile::T *>(base + bd.in_off), bd.ld,
+            reinterpret_cast<const int32_t *>(base + bd.table_off), bd.nchans, bd.nsamp, bd.ndms,
+            bd.row_words, cbr, reinterpret_cast<float *>(base + bd.out_off), tt, dt, lds);
+}
+
 // ------------------------------------------------------------ variant table
 // Variant v: F32 tile (MW, K, CB) and packed tiles (MW, K2, CB) with the same
 // TD = 4*MW and similar time extent.
@@ -599,6 +625,21 @@ size_t batch_lds_bytes(int dtype, int variant, uint32_t max_rw) {
     return MDSM_OK;
   });
   return bytes;
+}
+
+int launch_batch_flat(int dtype, int variant, const BeamDesc *d_beams, uint32_t nbeams,
+                      uint32_t total_items, uint32_t max_rw, hipStream_t s) {
+  MDSM_REQUIRE(max_rw > 0, MDSM_ERR_ARG, "delay spread too large for tile variant %d", variant);
+  return with_tile(dtype, variant, [&](auto *tp) -> int {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    const uint32_t cbr = stage_channels<Tile>(max_rw);
+    MDSM_REQUIRE(cbr >= 1, MDSM_ERR_ARG, "delay spread too large for LDS staging");
+    const size_t lds = (size_t)cbr * Tile::lds_words_per_row(max_rw) * sizeof(float);
+    hipLaunchKernelGGL(batch_flat_kernel<Tile>, dim3(total_items), dim3(256), lds, s, d_beams,
+                       nbeams, cbr);
+    MDSM_HIP(hipGetLastError());
+    return MDSM_OK;
+  });
 }
 
 int launch_batch_dedisperse(int dtype, int variant, const BeamDesc *d_beams, uint32_t nbeams,

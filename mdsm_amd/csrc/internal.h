@@ -79,6 +79,8 @@ struct BeamDesc {
   uint32_t pad;
 };
 
+int launch_batch_flat(int dtype, int variant, const BeamDesc *d_beams, uint32_t nbeams,
+                      uint32_t total_items, uint32_t max_rw, hipStream_t s);
 int launch_batch_dedisperse(int dtype, int variant, const BeamDesc *d_beams, uint32_t nbeams,
                             const uint32_t *d_order, uint32_t total_items, uint32_t *d_counter,
                             uint32_t max_row_words, int grid, hipStream_t stream);

@@ -1,6 +1,7 @@
-"""GPU parity of the persistent multi-beam dedisperser: every beam of an
-uneven batch (different nchans, nsamp, ndms, dmstep, band) must equal the
-oracle bit for bit, for F32, F16 and U8 storage."""
+"""GPU parity of the multi-beam dedisperser (persistent queue and flat
+XCD-ordered launch): every beam of an uneven batch (different nchans,
+nsamp, ndms, dmstep, band) must equal the oracle bit for bit, for F32, F16
+and U8 storage."""
 import sys
 
 import numpy as np
@@ -51,10 +52,15 @@ def test_batch_uneven_beams_bitexact(dtype):
         keep += [di, do]
         ins.append((di.ptr, ld))
         outs.append(do)
-    for v in (0, 1, 4):
-        batch.set_variant(v)
-        batch.dedisperse([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
-        abi.synchronize()
-        for b, s in enumerate(beams):
-            got = outs[b].download(np.float32, (s.ndms, s.nsamp))
-            assert np.array_equal(got, refs[b]), (v, b)
+    for mode in (abi.BATCH_PERSISTENT, abi.BATCH_FLAT):
+        batch.set_mode(mode)
+        for v in (0, 1, 4):
+            for b, s in enumerate(beams):  # every tile must be written again
+                outs[b].upload(np.full(s.ndms * s.nsamp, np.nan, np.float32))
+            print(f"[batch dtype {dtype}] mode {mode} variant {v}", flush=True, file=sys.stderr)
+            batch.set_variant(v)
+            batch.dedisperse([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
+            abi.synchronize()
+            for b, s in enumerate(beams):
+                got = outs[b].download(np.float32, (s.ndms, s.nsamp))
+                assert np.array_equal(got, refs[b]), (mode, v, b)
