@@ -1,12 +1,14 @@
 // batch.cpp -- multi-beam batching: many beams of uneven nchans / nsamp /
-// ndms dedispersed by one persistent launch that drains a device work queue.
+// ndms dedispersed by one launch: a persistent grid draining eight device
+// work queues (one per XCD), or a flat grid of one workgroup per tile.
 //
 // The reference runs one host pthread per beam, each launching
 // shared_dedispersion on its own slice of device memory
 // (src/mdsm-multibeam/multibeam_dedispersion_thread.cu:464-839,
 // multibeam_dedispersion_manager.cpp:495-512).  Here every beam's tiles are
-// items of one queue: a beam descriptor table in device memory, items
-// ordered largest beam first, one atomic counter.  mdsm_batch_launch issues
+// items: a beam descriptor table in device memory, beams ordered largest
+// first, items numbered so that the DM tiles sharing a time window land on
+// one XCD (and its L2), eight atomic counters.  mdsm_batch_launch issues
 // only a counter-clear kernel and the persistent kernel, so it can be
 // captured in a hipGraph and replayed per chunk.
 #include <algorithm>
@@ -104,7 +106,7 @@ extern "C" int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, in
     return (double)a.nchans * a.ndms * a.nsamp > (double)c.nchans * c.ndms * c.nsamp;
   });
   if (hipMalloc(&b->d_desc, sizeof(BeamDesc) * nbeams) != hipSuccess ||
-      hipMalloc(&b->d_counter, sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&b->d_counter, 8 * sizeof(uint32_t)) != hipSuccess) {
     set_error("batch descriptor allocation failed");
     free_batch(b);
     return MDSM_ERR_NOMEM;
@@ -171,9 +173,8 @@ extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const 
     d.n_dtiles = (s.ndms + sh.td() - 1) / sh.td();
     d.item_begin = (uint32_t)items;
     d.pad = 0;
-    // flat mode numbers items XCD-major: 8 slots per time-tile group
-    const uint64_t tgroups = batch->mode == MDSM_BATCH_FLAT ? (d.n_ttiles + 7u) / 8u * 8u : d.n_ttiles;
-    items += tgroups * d.n_dtiles;
+    // items are numbered XCD-major: 8 slots per group of 8 time tiles
+    items += (uint64_t)(d.n_ttiles + 7u) / 8u * 8u * d.n_dtiles;
     maxrw = std::max(maxrw, d.row_words);
   }
   MDSM_REQUIRE(items < (1ull << 31), MDSM_ERR_ARG, "too many work items");
@@ -202,7 +203,7 @@ extern "C" int mdsm_batch_launch(mdsm_batch batch, void *stream) {
     return launch_batch_flat(batch->dtype, (int)batch->bound_variant, batch->d_desc,
                              (uint32_t)batch->beams.size(), batch->total_items,
                              batch->max_row_words, s);
-  int rc = launch_zero_u32(batch->d_counter, 1, s);
+  int rc = launch_zero_u32(batch->d_counter, 8, s);  // one queue per XCD slot
   if (rc != MDSM_OK) return rc;
   return launch_batch_dedisperse(batch->dtype, (int)batch->bound_variant, batch->d_desc,
                                  (uint32_t)batch->beams.size(), nullptr, batch->total_items,

@@ -456,64 +456,80 @@ __global__ __launch_bounds__(256) void dedisp_kernel(const typename Tile::T *__r
   Tile::run(in, ld, table, nchans, nsamp, ndms, rw, cbr, out, tt, dt, lds);
 }
 
-// Persistent multi-beam kernel: workgroups pull items from a queue until it
-// is exhausted (every wave of every workgroup reaches the exit).  Items are
-// ordered by the host (largest beams first); item i belongs to the beam with
-// the largest item_begin <= i.
+// Multi-beam item numbering (both launch shapes): a beam's items are
+// XCD-major, 8 * ceil(n_ttiles / 8) * n_dtiles of them starting at a multiple
+// of 8; item g belongs to XCD slot g % 8 and decodes to time tile
+// 8 * (k / n_dtiles) + g % 8, DM tile k % n_dtiles, k = (g - begin) / 8
+// (padding items with a time tile past the beam's end do nothing).
+struct BeamItem {
+  const BeamDesc *bd;
+  uint32_t tt, dt;
+  bool valid;
+};
+
+__device__ __forceinline__ BeamItem decode_item(const BeamDesc *__restrict__ beams, uint32_t nbeams,
+                                                uint32_t g) {
+  uint32_t lo = 0, hi = nbeams - 1u;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    if (beams[mid].item_begin <= g) lo = mid;
+    else hi = mid - 1u;
+  }
+  const BeamDesc *bd = beams + lo;
+  const uint32_t local = g - bd->item_begin;
+  const uint32_t k = local >> 3;
+  BeamItem it;
+  it.bd = bd;
+  it.tt = ((k / bd->n_dtiles) << 3) + (local & 7u);
+  it.dt = k % bd->n_dtiles;
+  it.valid = it.tt < bd->n_ttiles;
+  return it;
+}
+
+template <typename Tile>
+__device__ __forceinline__ void run_item(const BeamDesc *__restrict__ beams, const BeamItem &it,
+                                         uint32_t cbr, float *lds) {
+  const BeamDesc &bd = *it.bd;
+  char *base = reinterpret_cast<char *>(const_cast<BeamDesc *>(beams));
+  Tile::run(reinterpret_cast<const typename Tile::T *>(base + bd.in_off), bd.ld,
+            reinterpret_cast<const int32_t *>(base + bd.table_off), bd.nchans, bd.nsamp, bd.ndms,
+            bd.row_words, cbr, reinterpret_cast<float *>(base + bd.out_off), it.tt, it.dt, lds);
+}
+
+// Persistent multi-beam kernel: resident workgroups pull items from eight
+// queues, one per XCD slot (workgroup b drains queue b % 8, the XCD the
+// dispatcher puts it on), until the queue passes total_items; every wave
+// of every workgroup reaches the exit.  Beams are ordered by the host,
+// largest first.
 template <typename Tile>
 __global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__ beams,
                                                     uint32_t nbeams, uint32_t total_items,
                                                     uint32_t cbr, uint32_t *__restrict__ counter) {
   extern __shared__ float lds[];
   __shared__ uint32_t item_s;
+  const uint32_t slot = blockIdx.x & 7u;
   while (true) {
-    if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) item_s = atomicAdd(counter + slot, 1u) * 8u + slot;
     __syncthreads();
     // wave-uniform: the descriptor lookup and every tile parameter stay scalar
     const uint32_t item = uniform(item_s);
     __syncthreads();
     if (item >= total_items) break;
-    uint32_t lo = 0, hi = nbeams - 1u;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1u) >> 1;
-      if (beams[mid].item_begin <= item) lo = mid;
-      else hi = mid - 1u;
-    }
-    const BeamDesc &bd = beams[lo];
-    const uint32_t local = item - bd.item_begin;
-    char *base = reinterpret_cast<char *>(const_cast<BeamDesc *>(beams));
-    Tile::run(reinterpret_cast<const typename Tile::T *>(base + bd.in_off), bd.ld,
-              reinterpret_cast<const int32_t *>(base + bd.table_off), bd.nchans, bd.nsamp,
-              bd.ndms, bd.row_words, cbr, reinterpret_cast<float *>(base + bd.out_off),
-              local / bd.n_dtiles, local % bd.n_dtiles, lds);
+    const BeamItem it = decode_item(beams, nbeams, item);
+    if (it.valid) run_item<Tile>(beams, it, cbr, lds);
     __syncthreads();
   }
 }
 
-// Flat multi-beam kernel: one workgroup per item, all beams in one launch.
-// Items of a beam are numbered XCD-major as in dedisp_kernel (its item
-// count is 8 * ceil(n_ttiles / 8) * n_dtiles and item_begin is a multiple
-// of 8, so blockIdx % 8 is the item's XCD slot).
+// Flat multi-beam kernel: one workgroup per item (workgroup ID = item, so
+// the dispatcher's round-robin puts item g on XCD g % 8).
 template <typename Tile>
 __global__ __launch_bounds__(256) void batch_flat_kernel(const BeamDesc *__restrict__ beams,
                                                          uint32_t nbeams, uint32_t cbr) {
   extern __shared__ float lds[];
-  const uint32_t item = blockIdx.x;
-  uint32_t lo = 0, hi = nbeams - 1u;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1u) >> 1;
-    if (beams[mid].item_begin <= item) lo = mid;
-    else hi = mid - 1u;
-  }
-  const BeamDesc &bd = beams[lo];
-  const uint32_t local = item - bd.item_begin;
-  const uint32_t x = local & 7u, k = local >> 3;
-  const uint32_t tt = ((k / bd.n_dtiles) << 3) + x, dt = k % bd.n_dtiles;
-  if (tt >= bd.n_ttiles) return;  // whole workgroup, before any barrier
-  char *base = reinterpret_cast<char *>(const_cast<BeamDesc *>(beams));
-  Tile::run(reinterpret_cast<const typename Tile::T *>(base + bd.in_off), bd.ld,
-            reinterpret_cast<const int32_t *>(base + bd.table_off), bd.nchans, bd.nsamp, bd.ndms,
-            bd.row_words, cbr, reinterpret_cast<float *>(base + bd.out_off), tt, dt, lds);
+  const BeamItem it = decode_item(beams, nbeams, blockIdx.x);
+  if (!it.valid) return;  // whole workgroup, before any barrier
+  run_item<Tile>(beams, it, cbr, lds);
 }
 
 // ------------------------------------------------------------ variant table
@@ -659,6 +675,8 @@ int launch_batch_dedisperse(int dtype, int variant, const BeamDesc *d_beams, uin
       MDSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
       grid = std::max(1, per_cu) * std::max(1, cus);
     }
+    // every one of the eight queues needs workers: a multiple of 8, at least 8
+    grid = std::max(8, (grid + 7) / 8 * 8);
     hipLaunchKernelGGL(batch_kernel<Tile>, dim3(grid), dim3(256), lds, s, d_beams, nbeams,
                        total_items, cbr, d_counter);
     MDSM_HIP(hipGetLastError());
