@@ -113,7 +113,11 @@ __device__ __forceinline__ const T *at_bytes(const T *p, int32_t bytes) {
 }
 
 // ======================================================================= F32
-template <int MW, int K, int CB_>
+// BURST > 0: the LDS reads of BURST trials are issued together ahead of
+// their adds (a scheduling fence keeps them grouped), so more reads are in
+// flight per wave at the cost of BURST*K registers; 0 leaves the order to
+// the compiler.
+template <int MW, int K, int CB_, int BURST = 0>
 struct F32Tile {
   static constexpr int TT = 64 * K;
   static constexpr int TD = 4 * MW;
@@ -179,11 +183,29 @@ struct F32Tile {
 #pragma unroll
       for (int r = 0; r < CB; ++r) {
         if ((uint32_t)r < nc) {
+          if constexpr (BURST > 0) {
 #pragma unroll
-          for (int i = 0; i < MW; ++i) {
-            const float *p = at_bytes(cur, rel.get(r, i));
+            for (int i0 = 0; i0 < MW; i0 += BURST) {
+              float w[BURST][K];
 #pragma unroll
-            for (int j = 0; j < K; ++j) acc[i][j] += p[64 * j];
+              for (int g = 0; g < BURST; ++g) {
+                const float *p = at_bytes(cur, rel.get(r, i0 + g));
+#pragma unroll
+                for (int j = 0; j < K; ++j) w[g][j] = p[64 * j];
+              }
+              __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+              for (int g = 0; g < BURST; ++g)
+#pragma unroll
+                for (int j = 0; j < K; ++j) acc[i0 + g][j] += w[g][j];
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < MW; ++i) {
+              const float *p = at_bytes(cur, rel.get(r, i));
+#pragma unroll
+              for (int j = 0; j < K; ++j) acc[i][j] += p[64 * j];
+            }
           }
         }
       }
@@ -230,7 +252,7 @@ struct PackedGeom {
 // loads while the previous block is added; after the barrier one LDS->LDS
 // pass builds the pair copies (F16: P is the copy itself and Q its 16-bit
 // funnel shift; U8: bytes widened to u16 pairs with v_perm into P and Q).
-template <int MODE, int MW, int K2, int CB_>
+template <int MODE, int MW, int K2, int CB_, int BURST = 0>
 struct PackedTile : PackedGeom<MW, K2, CB_> {
   using G = PackedGeom<MW, K2, CB_>;
   static constexpr int TT = G::TT, TD = G::TD, CB = CB_;
@@ -358,22 +380,40 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
         }
         since_fold += nc;
       }
+      auto add_word = [&](int i, int j, uint32_t word) {
+        if (IS_U8) {
+          uacc[i][j] += __builtin_bit_cast(ushort2_t, word);
+        } else {
+          const half2_t h = __builtin_bit_cast(half2_t, word);
+          facc[i][j][0] = __builtin_fmaf((float)h.x, one, facc[i][j][0]);
+          facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
+        }
+      };
 #pragma unroll
       for (int r = 0; r < CB; ++r) {
         if ((uint32_t)r < nc) {
+          if constexpr (BURST > 0) {
 #pragma unroll
-          for (int i = 0; i < MW; ++i) {
-            const uint32_t *p = at_bytes(cur, rel.get(r, i));
+            for (int i0 = 0; i0 < MW; i0 += BURST) {
+              uint32_t w[BURST][K2];
 #pragma unroll
-            for (int j = 0; j < K2; ++j) {
-              const uint32_t word = p[64 * j];
-              if (IS_U8) {
-                uacc[i][j] += __builtin_bit_cast(ushort2_t, word);
-              } else {
-                const half2_t h = __builtin_bit_cast(half2_t, word);
-                facc[i][j][0] = __builtin_fmaf((float)h.x, one, facc[i][j][0]);
-                facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
+              for (int g = 0; g < BURST; ++g) {
+                const uint32_t *p = at_bytes(cur, rel.get(r, i0 + g));
+#pragma unroll
+                for (int j = 0; j < K2; ++j) w[g][j] = p[64 * j];
               }
+              __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+              for (int g = 0; g < BURST; ++g)
+#pragma unroll
+                for (int j = 0; j < K2; ++j) add_word(i0 + g, j, w[g][j]);
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < MW; ++i) {
+              const uint32_t *p = at_bytes(cur, rel.get(r, i));
+#pragma unroll
+              for (int j = 0; j < K2; ++j) add_word(i, j, p[64 * j]);
             }
           }
         }
@@ -413,10 +453,10 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
   }
 };
 
-template <int MW, int K2, int CB>
-using F16Tile = PackedTile<MDSM_F16, MW, K2, CB>;
-template <int MW, int K2, int CB>
-using U8Tile = PackedTile<MDSM_U8, MW, K2, CB>;
+template <int MW, int K2, int CB, int BURST = 0>
+using F16Tile = PackedTile<MDSM_F16, MW, K2, CB, BURST>;
+template <int MW, int K2, int CB, int BURST = 0>
+using U8Tile = PackedTile<MDSM_U8, MW, K2, CB, BURST>;
 
 // XCD-aware workgroup order (on unless MDSM_XCD_REMAP=0 in the environment)
 bool xcd_remap_enabled() {
@@ -542,7 +582,11 @@ template <> struct VariantDef<2> { using F32 = F32Tile<16, 4, 4>;  using F16 = F
 template <> struct VariantDef<3> { using F32 = F32Tile<8, 8, 8>;   using F16 = F16Tile<8, 4, 8>;   using U8 = U8Tile<8, 4, 8>; };
 template <> struct VariantDef<4> { using F32 = F32Tile<16, 8, 4>;  using F16 = F16Tile<16, 4, 4>;  using U8 = U8Tile<16, 4, 4>; };
 template <> struct VariantDef<5> { using F32 = F32Tile<32, 2, 2>;  using F16 = F16Tile<32, 1, 2>;  using U8 = U8Tile<32, 1, 2>; };
-constexpr int kNumVariants = 5;
+// grouped-read forms of 2 and 1
+template <> struct VariantDef<6> { using F32 = F32Tile<16, 4, 4, 8>; using F16 = F16Tile<16, 2, 4, 16>; using U8 = U8Tile<16, 2, 4, 16>; };
+template <> struct VariantDef<7> { using F32 = F32Tile<8, 4, 8, 8>;  using F16 = F16Tile<8, 2, 8, 8>;   using U8 = U8Tile<8, 2, 8, 8>; };
+template <> struct VariantDef<8> { using F32 = F32Tile<16, 4, 4, 4>; using F16 = F16Tile<16, 2, 4, 8>;  using U8 = U8Tile<16, 2, 4, 8>; };
+constexpr int kNumVariants = 8;
 
 template <typename V, typename Fn>
 int pick_dtype(int dtype, Fn &&fn) {
@@ -563,6 +607,9 @@ int with_tile(int dtype, int variant, Fn &&fn) {
     case 3: return pick_dtype<VariantDef<3>>(dtype, fn);
     case 4: return pick_dtype<VariantDef<4>>(dtype, fn);
     case 5: return pick_dtype<VariantDef<5>>(dtype, fn);
+    case 6: return pick_dtype<VariantDef<6>>(dtype, fn);
+    case 7: return pick_dtype<VariantDef<7>>(dtype, fn);
+    case 8: return pick_dtype<VariantDef<8>>(dtype, fn);
   }
   set_error("unknown dedispersion variant %d", variant);
   return MDSM_ERR_ARG;
