@@ -66,17 +66,32 @@ def build_stream(cfg, seed, max_events, max_chunks=100000, log=print):
     return d_snip, events[:n], stats, time.time() - t0
 
 
+THRESHOLDS = (0.9, 0.5, 0.3, 0.2, 0.1, 0.05, 0.02, 0.01, 0.005, 0.002, 0.001)
+
+
+def recall_fp(prob, events, stats, t):
+    """Recall on injected pulses and false-positive count of the events kept at prob > t."""
+    keep = prob > t
+    hits = keep & (events["label"] == 1)
+    found = len(set(zip(events["chunk"][hits].tolist(), events["pulse"][hits].tolist())))
+    return found / max(stats["injected"], 1), int(((events["label"] == 0) & keep).sum())
+
+
+def operating_threshold(prob, events, stats, fp_fraction=0.25):
+    """Lowest threshold whose false positives stay under fp_fraction of the
+    plain path's on a validation stream (lower thresholds keep more pulses)."""
+    budget = fp_fraction * stats["false_clusters"]
+    ok = [t for t in THRESHOLDS if recall_fp(prob, events, stats, t)[1] <= budget]
+    return min(ok) if ok else max(THRESHOLDS)
+
+
 def judge(prob, events, stats, thresholds):
     """Plain vs learned triage on one held-out stream."""
     out = {"plain": {"recall": stats["injected_found"] / stats["injected"], "false_positives":
                      stats["false_clusters"], "clusters": stats["clusters"]}}
     for t in thresholds:
-        keep = prob > t
-        hits = keep & (events["label"] == 1)
-        found = len(set(zip(events["chunk"][hits].tolist(), events["pulse"][hits].tolist())))
-        out[f"cnn@{t}"] = {"recall": found / stats["injected"],
-                           "false_positives": int(((events["label"] == 0) & keep).sum()),
-                           "kept": int(keep.sum())}
+        recall, fp = recall_fp(prob, events, stats, t)
+        out[f"cnn@{t}"] = {"recall": recall, "false_positives": fp, "kept": int((prob > t).sum())}
     return out
 
 
@@ -88,7 +103,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--threshold", type=float, default=0.2)
+    ap.add_argument("--threshold", type=float, default=None,
+                    help="operating threshold (default: picked on a validation stream)")
     ap.add_argument("--save", default=None, help="write trained parameters (.npy, canonical layout)")
     ap.add_argument("--json", default=None, help="write the results as JSON")
     args = ap.parse_args(argv)
@@ -111,7 +127,19 @@ def main(argv=None):
         res["epochs"].append({"loss": loss, "seconds": dt, "events_per_s": len(ev_tr) / dt})
         log(f"epoch {ep}: loss {loss:.4f}, {len(ev_tr) / dt:.0f} events/s")
     del d_tr
-    thresholds = sorted({0.1, args.threshold, 0.5, 0.9})
+    # operating point from a validation stream of the training survey
+    d_va, ev_va, st_va, _ = build_stream(make_config(TRAIN_SURVEY), args.seed + 50, 400 * args.eval_chunks,
+                                         max_chunks=max(args.eval_chunks // 2, 1), log=log)
+    d_prob = abi.DeviceBuffer(4 * max(len(ev_va), 1))
+    abi.predict(net, d_va.ptr, len(ev_va), d_prob.ptr)
+    abi.synchronize()
+    pv = d_prob.download(np.float32, (len(ev_va),))
+    thr = args.threshold if args.threshold is not None else operating_threshold(pv, ev_va, st_va)
+    res["validation"] = judge(pv, ev_va, st_va, THRESHOLDS)
+    res["threshold"] = thr
+    log(f"operating threshold {thr}")
+    del d_va, d_prob
+    thresholds = sorted({0.1, thr, 0.5, 0.9})
     res["eval"] = []
     for i, sv in enumerate(EVAL_SURVEYS):
         d_te, ev_te, st_te, _ = build_stream(make_config(sv), args.seed + 100 + i, 400 * args.eval_chunks,
@@ -125,6 +153,7 @@ def main(argv=None):
         prob = d_prob.download(np.float32, (n,))
         r = judge(prob, ev_te, st_te, thresholds)
         r["survey"] = sv
+        r["operating_point"] = r[f"cnn@{thr}"]
         r["inference_events_per_s"] = n / max(t_inf, 1e-9)
         res["eval"].append(r)
         log(json.dumps(r))

@@ -1,13 +1,15 @@
 """Triage data on the GPU: RFI injection and snippet extraction against the
 oracle (bit exact), the synthetic search stream (labels, statistics), and
 the learned triage against the plain threshold + DBSCAN path on held-out
-chunks: it keeps the plain path's recall on injected pulses (within 0.03)
-at no more than half of its false positives."""
+chunks: at an operating threshold picked on a separate validation stream,
+it keeps the plain path's recall on injected pulses (within 0.03) at no more
+than half of its false positives."""
 import numpy as np
 import pytest
 
 import oracle
 from mdsm_amd import abi
+from mdsm_amd.triage_train import THRESHOLDS, operating_threshold, recall_fp
 
 pytestmark = pytest.mark.gpu
 
@@ -87,13 +89,6 @@ def test_search_stream_statistics():
     assert (centre > edge).mean() > 0.9
 
 
-def _recall_fp(prob, ev, st, thr):
-    keep = prob > thr
-    hits = keep & (ev["label"] == 1)
-    recall = len(set(zip(ev["chunk"][hits], ev["pulse"][hits]))) / st["injected"]
-    return recall, int(((ev["label"] == 0) & keep).sum())
-
-
 def _predict(net, d, n):
     d_prob = abi.DeviceBuffer(4 * n)
     abi.predict(net, d.ptr, n, d_prob.ptr)
@@ -111,14 +106,16 @@ def test_learned_triage_beats_plain_false_positives():
     for ep in range(8):
         loss = abi.train_epoch(net, d_tr.ptr, lab.ptr, len(ev_tr), 128, 1e-3, 100 + ep)
     print("final training loss", loss, flush=True)
-    # the largest threshold that keeps the plain path's recall on the validation stream
+    # operating point from the validation stream: the lowest threshold whose
+    # false positives stay under a quarter of the plain path's there (a lower
+    # threshold can only keep more of the injected pulses)
     pv = _predict(net, d_va, len(ev_va))
-    plain_va = st_va["injected_found"] / st_va["injected"]
-    thr = next(t for t in (0.5, 0.3, 0.2, 0.1, 0.05, 0.02, 0.01, 0.005, 0.0)
-               if _recall_fp(pv, ev_va, st_va, t)[0] >= plain_va - 0.005)
+    print("validation (recall, fp) per threshold:", {t: recall_fp(pv, ev_va, st_va, t) for t in THRESHOLDS},
+          "plain fp", st_va["false_clusters"], flush=True)
+    thr = operating_threshold(pv, ev_va, st_va)
     prob = _predict(net, d_te, len(ev_te))
     net.close()
-    recall, fp = _recall_fp(prob, ev_te, st_te, thr)
+    recall, fp = recall_fp(prob, ev_te, st_te, thr)
     plain_recall = st_te["injected_found"] / st_te["injected"]
     print(f"threshold {thr}: plain recall {plain_recall:.3f} fp {st_te['false_clusters']}; "
           f"cnn recall {recall:.3f} fp {fp}")
