@@ -346,6 +346,8 @@ extern "C" int mdsm_dedisperse(mdsm_plan plan, const void *d_in, uint64_t ld, fl
   MDSM_REQUIRE(ld % input_alignment(plan->dtype) == 0 && ((uintptr_t)d_in & 3u) == 0, MDSM_ERR_ARG,
                "input rows must start on 4-byte boundaries (row length a multiple of %u)",
                input_alignment(plan->dtype));
+  MDSM_REQUIRE(ld / input_alignment(plan->dtype) < (1ull << 29), MDSM_ERR_ARG,
+               "input rows must be shorter than 2 GiB");
   const int v = plan_variant(plan);
   return launch_dedisperse(plan->dtype, v, d_in, ld, plan->d_table, plan->s.nchans, plan->s.nsamp,
                            plan->s.ndms, plan->row_words[v], d_out, as_stream(stream));

@@ -157,6 +157,8 @@ extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const 
                  "beam %u: input row length below nsamp + maxshift", i);
     MDSM_REQUIRE(ld[i] % input_alignment(batch->dtype) == 0 && ((uintptr_t)d_in[i] & 3u) == 0,
                  MDSM_ERR_ARG, "beam %u: input rows must start on 4-byte boundaries", i);
+    MDSM_REQUIRE(ld[i] / input_alignment(batch->dtype) < (1ull << 29), MDSM_ERR_ARG,
+                 "beam %u: input rows must be shorter than 2 GiB", i);
     MDSM_REQUIRE(batch->row_words[i][v] > 0, MDSM_ERR_ARG,
                  "beam %u: delay spread too large for tile variant %d", i, v);
     BeamDesc &d = batch->h_desc[k];

@@ -14,8 +14,8 @@
 // for K LDS reads (issued in pairs as ds_read2st64).  The adds are bound by
 // LDS read bandwidth (~128 B/clk/CU), not by HBM, so the storage formats
 // trade LDS bytes per add:
-//   F32  4 B per add: one float per LDS word, staged by direct global->LDS
-//        loads (global_load_lds_dword) that use no VGPRs.
+//   F32  4 B per add: one float per LDS word, staged by direct buffer->LDS
+//        loads (buffer_load_dword ... lds) that use no data VGPRs.
 //   F16  2 B per add: LDS words hold pairs of consecutive samples in two
 //        copies (even and odd alignment), so any wave-uniform offset reads
 //        an aligned word; v_fma_mix_f32 (half operand selected by op_sel,
@@ -52,6 +52,24 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgc
 
 // s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (GFX9 encoding).
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// One 64-dword chunk from global memory straight into LDS: lane l copies the
+// dword at src + 4 l to lds_dst + 4 l (rows are shorter than 2 GiB).  The copy goes through a raw buffer
+// resource whose base is src and whose size is `avail` bytes, so lanes at
+// or past `avail` read 0 instead of touching memory (the hardware range
+// check replaces a per-lane compare), and the only per-lane operand is the
+// constant 4*lane (no 64-bit address arithmetic in VGPRs).  Dword 3 of the
+// resource (0x00020000: 32-bit data format, raw addressing) is the GFX9
+// family value.
+__device__ __forceinline__ void chunk_to_lds(const void *src, uint32_t avail, void *lds_dst,
+                                             uint32_t lane4) {
+  // the operands are wave-uniform; readfirstlane keeps the resource in SGPRs
+  const uint64_t a = reinterpret_cast<uint64_t>(src);
+  const uint64_t au = ((uint64_t)uniform((uint32_t)(a >> 32)) << 32) | (uint64_t)uniform((uint32_t)a);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      reinterpret_cast<void *>(au), (short)0, (int)uniform(avail), 0x00020000);
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)lds_dst, 4, (int)lane4, 0, 0, 0);
+}
 
 // LDS byte offsets of the wave's trials inside the windows of one channel
 // block, relative to the block buffer plus the lane's own word.  Lane l of
@@ -141,8 +159,9 @@ struct F32Tile {
     const uint32_t bufw = cbr * rw;  // words per LDS buffer (offsets keep the LDS address space)
 
     // wave w copies 64-word chunks w, w+4, ... of each row from global
-    // memory straight into LDS; lanes past the end of the row stay idle
-    // (their words only feed outputs beyond nsamp)
+    // memory straight into LDS; words past the end of the row read as 0
+    // (they only feed outputs beyond nsamp)
+    const uint32_t lane4 = 4u * lane;
     auto stage = [&](uint32_t cb, float *b) {
       const uint32_t nc = min(cbr, nchans - cb);
       for (uint32_t r = 0; r < nc; ++r) {
@@ -151,11 +170,14 @@ struct F32Tile {
         const uint32_t len = (uint32_t)TT + (uint32_t)(trow[dlast] - base);
         const uint64_t g0 = (uint64_t)t0 + (uint32_t)base;
         const float *src = in + (uint64_t)(cb + r) * ld + g0;
+        // bytes to the row end (rows are shorter than 2 GiB, so signed 32-bit
+        // offsets hold them and the clamps stay scalar)
+        const int32_t ld32 = (int32_t)ld;
+        const int32_t avail = 4 * (ld32 - min((int32_t)g0, ld32));
         const uint32_t nq = (len + 63u) >> 6;
         for (uint32_t q = wave; q < nq; q += 4u) {
-          if (g0 + 64u * q + lane < ld)
-            __builtin_amdgcn_global_load_lds((glb_void *)(src + 64u * q + lane),
-                                             (lds_void *)(b + r * rw + 64u * q), 4, 0, 0);
+          const int32_t rest = max(avail - 256 * (int32_t)q, 0);
+          chunk_to_lds(src + 64u * q, (uint32_t)rest, b + r * rw + 64u * q, lane4);
         }
       }
     };
@@ -286,19 +308,22 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
 
     auto g0_of = [&](uint32_t c) { return t0 + (uint32_t)table[(uint64_t)c * ndms + d0]; };
     // copy the window dwords of rows [cb, cb+nc) into P of buffer b (F16) or
-    // into the raw area (U8); wave w takes 64-dword chunks w, w+4, ...
+    // into the raw area (U8); wave w takes 64-dword chunks w, w+4, ...;
+    // dwords past the end of the row read as 0
+    const uint32_t lane4 = 4u * lane;
     auto stage = [&](uint32_t cb, uint32_t *b) {
       const uint32_t nc = min(cbr, nchans - cb);
       for (uint32_t r = 0; r < nc; ++r) {
         const uint32_t g0 = g0_of(cb + r);
         const uint64_t w0 = IS_U8 ? (g0 >> 2) : (g0 >> 1);
         const uint32_t *src = src32 + (uint64_t)(cb + r) * ldw + w0;
+        const int32_t ldw32 = (int32_t)ldw;  // rows < 2 GiB
+        const int32_t avail = 4 * (ldw32 - min((int32_t)w0, ldw32));
         uint32_t *dst = IS_U8 ? raw + r * rawn : b + r * rw;
         const uint32_t n = IS_U8 ? rawn : wp;
         for (uint32_t q = wave; 64u * q < n; q += 4u) {
-          if (w0 + 64u * q + lane < ldw)
-            __builtin_amdgcn_global_load_lds((glb_void *)(src + 64u * q + lane),
-                                             (lds_void *)(dst + 64u * q), 4, 0, 0);
+          const int32_t rest = max(avail - 256 * (int32_t)q, 0);
+          chunk_to_lds(src + 64u * q, (uint32_t)rest, dst + 64u * q, lane4);
         }
       }
     };
