@@ -241,12 +241,37 @@ struct mdsm_plan_s {
   int32_t *d_table = nullptr;
   int variant = 0;  // 0 until set or tuned; then 1..num_variants
   std::vector<uint32_t> row_words;  // per variant (index 1..)
+  std::vector<int32_t *> d_rel;     // trial-offset table per variant, built on first use
 };
+
+// Build the trial-offset table of variant v on stream s if it is missing
+// (one allocation and one kernel, the first time the variant is used).
+static int ensure_trial_offsets(mdsm_plan_s *p, int v, hipStream_t s) {
+  if (p->d_rel[v]) return MDSM_OK;
+  const uint32_t rw = p->row_words[v];
+  const uint32_t cbr = stage_channels(p->dtype, v, rw);
+  MDSM_REQUIRE(rw > 0 && cbr > 0, MDSM_ERR_ARG,
+               "delay spread too large for tile variant %d (choose another variant)", v);
+  const uint64_t words = trial_offset_words(p->dtype, v, p->s.nchans, p->s.ndms, cbr);
+  int32_t *d = nullptr;
+  hipError_t e = hipMalloc(&d, words * sizeof(int32_t));
+  if (e != hipSuccess) {
+    set_error("hipMalloc(trial offsets) failed: %s", hipGetErrorString(e));
+    return MDSM_ERR_NOMEM;
+  }
+  const int rc = launch_trial_offsets(p->dtype, v, p->d_table, p->s.nchans, p->s.ndms, rw, cbr, d, s);
+  if (rc != MDSM_OK) {
+    (void)hipFree(d);
+    return rc;
+  }
+  p->d_rel[v] = d;
+  return MDSM_OK;
+}
 
 // Untuned default: the preferred tile shape if it can stage this survey's
 // delay spread, else the first one that can.
 static int default_variant(const mdsm_plan_s *p) {
-  const int pref = 3;
+  const int pref = preferred_variant(p->dtype);
   if (p->row_words[pref]) return pref;
   for (int v = 1; v < (int)p->row_words.size(); ++v)
     if (p->row_words[v]) return v;
@@ -271,6 +296,7 @@ extern "C" int mdsm_plan_create(const mdsm_survey_t *s, int in_dtype, mdsm_plan 
   }
   const int nv = num_variants(in_dtype);
   p->row_words.assign(nv + 1, 0);
+  p->d_rel.assign(nv + 1, nullptr);
   for (int v = 1; v <= nv; ++v) {
     const TileShape sh = variant_shape(in_dtype, v);
     p->row_words[v] =
@@ -296,6 +322,8 @@ extern "C" int mdsm_plan_create(const mdsm_survey_t *s, int in_dtype, mdsm_plan 
 
 extern "C" int mdsm_plan_destroy(mdsm_plan plan) {
   if (!plan) return MDSM_OK;
+  for (int32_t *d : plan->d_rel)
+    if (d) MDSM_HIP(hipFree(d));
   if (plan->d_table) MDSM_HIP(hipFree(plan->d_table));
   delete plan;
   return MDSM_OK;
@@ -328,6 +356,14 @@ extern "C" int mdsm_plan_set_variant(mdsm_plan plan, int variant) {
   MDSM_REQUIRE(variant >= 0 && variant <= num_variants(plan->dtype), MDSM_ERR_ARG,
                "variant %d out of range", variant);
   plan->variant = variant;
+  // build the variant's trial-offset table now, so that dedisperse calls
+  // (and graphs captured from them) allocate nothing
+  const int v = plan_variant(plan);
+  if (plan->row_words[v] && stage_channels(plan->dtype, v, plan->row_words[v])) {
+    const int rc = ensure_trial_offsets(plan, v, nullptr);
+    if (rc != MDSM_OK) return rc;
+    MDSM_HIP(hipStreamSynchronize(nullptr));
+  }
   return MDSM_OK;
 }
 
@@ -349,8 +385,11 @@ extern "C" int mdsm_dedisperse(mdsm_plan plan, const void *d_in, uint64_t ld, fl
   MDSM_REQUIRE(ld / input_alignment(plan->dtype) < (1ull << 29), MDSM_ERR_ARG,
                "input rows must be shorter than 2 GiB");
   const int v = plan_variant(plan);
-  return launch_dedisperse(plan->dtype, v, d_in, ld, plan->d_table, plan->s.nchans, plan->s.nsamp,
-                           plan->s.ndms, plan->row_words[v], d_out, as_stream(stream));
+  const int rc = ensure_trial_offsets(plan, v, as_stream(stream));
+  if (rc != MDSM_OK) return rc;
+  return launch_dedisperse(plan->dtype, v, d_in, ld, plan->d_table, plan->d_rel[v], plan->s.nchans,
+                           plan->s.nsamp, plan->s.ndms, plan->row_words[v], d_out,
+                           as_stream(stream));
 }
 
 extern "C" int mdsm_plan_autotune(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out,

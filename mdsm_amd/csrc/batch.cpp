@@ -30,6 +30,7 @@ struct mdsm_batch_s {
   std::vector<std::vector<int32_t>> h_tables;
   std::vector<int32_t *> d_tables;
   std::vector<std::vector<uint32_t>> row_words;  // [beam][variant]
+  std::vector<std::vector<int32_t *>> d_rel;     // [beam][variant] trial-offset tables
   std::vector<uint32_t> order;                   // processing order of beams
   BeamDesc *d_desc = nullptr;
   uint32_t *d_counter = nullptr;
@@ -49,15 +50,19 @@ static int batch_variant(const mdsm_batch_s *b) {
       if (!rw[v]) return false;
     return true;
   };
-  if (fits(3)) return 3;
+  const int pref = preferred_variant(b->dtype);
+  if (fits(pref)) return pref;
   for (int v = 1; v <= num_variants(b->dtype); ++v)
     if (fits(v)) return v;
-  return 3;
+  return pref;
 }
 
 static void free_batch(mdsm_batch_s *b) {
   for (auto *p : b->d_tables)
     if (p) (void)hipFree(p);
+  for (auto &per : b->d_rel)
+    for (auto *p : per)
+      if (p) (void)hipFree(p);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_counter) (void)hipFree(b->d_counter);
   delete b;
@@ -77,6 +82,7 @@ extern "C" int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, in
   b->d_tables.assign(nbeams, nullptr);
   b->row_words.resize(nbeams);
   const int nv = num_variants(in_dtype);
+  b->d_rel.assign(nbeams, std::vector<int32_t *>(nv + 1, nullptr));
   for (uint32_t i = 0; i < nbeams; ++i) {
     int rc = build_delay_table(&beams[i], b->h_tables[i], &b->maxshift[i]);
     if (rc != MDSM_OK) {
@@ -149,6 +155,27 @@ extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const 
   batch->h_desc.resize(nb);
   uint64_t items = 0;
   uint32_t maxrw = 0;
+  for (uint32_t i = 0; i < nb; ++i) maxrw = std::max(maxrw, batch->row_words[i][v]);
+  // one launch stages every beam with the same channel block (from the
+  // widest row), so each beam's trial-offset table is built for that block
+  const uint32_t cbr = stage_channels(batch->dtype, v, maxrw);
+  for (uint32_t i = 0; i < nb; ++i) {
+    const auto &s = batch->beams[i];
+    MDSM_REQUIRE(batch->row_words[i][v] > 0 && cbr > 0, MDSM_ERR_ARG,
+                 "beam %u: delay spread too large for tile variant %d", i, v);
+    if (batch->d_rel[i][v]) continue;
+    const uint64_t words = trial_offset_words(batch->dtype, v, s.nchans, s.ndms, cbr);
+    int32_t *d = nullptr;
+    MDSM_HIP(hipMalloc(&d, words * sizeof(int32_t)));
+    const int rc = launch_trial_offsets(batch->dtype, v, batch->d_tables[i], s.nchans, s.ndms,
+                                        batch->row_words[i][v], cbr, d, nullptr);
+    if (rc != MDSM_OK) {
+      (void)hipFree(d);
+      return rc;
+    }
+    batch->d_rel[i][v] = d;
+  }
+  MDSM_HIP(hipStreamSynchronize(nullptr));
   for (uint32_t k = 0; k < nb; ++k) {
     const uint32_t i = batch->order[k];
     const auto &s = batch->beams[i];
@@ -166,6 +193,7 @@ extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const 
     d.in_off = reinterpret_cast<intptr_t>(d_in[i]) - base;
     d.out_off = reinterpret_cast<intptr_t>(d_out[i]) - base;
     d.table_off = reinterpret_cast<intptr_t>(batch->d_tables[i]) - base;
+    d.rel_off = reinterpret_cast<intptr_t>(batch->d_rel[i][v]) - base;
     d.ld = ld[i];
     d.nchans = s.nchans;
     d.nsamp = s.nsamp;
@@ -177,7 +205,6 @@ extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const 
     d.pad = 0;
     // items are numbered XCD-major: 8 slots per group of 8 time tiles
     items += (uint64_t)(d.n_ttiles + 7u) / 8u * 8u * d.n_dtiles;
-    maxrw = std::max(maxrw, d.row_words);
   }
   MDSM_REQUIRE(items < (1ull << 31), MDSM_ERR_ARG, "too many work items");
   MDSM_HIP(hipMemcpy(batch->d_desc, batch->h_desc.data(), sizeof(BeamDesc) * nb,

@@ -83,31 +83,38 @@ __device__ __forceinline__ void chunk_to_lds(const void *src, uint32_t avail, vo
 // (copy P is the first half of a row, copy Q the second).  Computed once per
 // block; a trial's address is then one v_add_u32 whose offset operand is
 // broadcast from lane i of each row by DPP row_newbcast (no v_readlane).
+template <int MW, int PACK>
+__device__ __forceinline__ int32_t rel_entry(const int32_t *__restrict__ table, uint32_t ndms,
+                                             uint32_t nchans, uint32_t cb, uint32_t d0,
+                                             uint32_t dw, uint32_t idx, uint32_t rw) {
+  const uint32_t r = idx / MW;
+  const uint32_t c = min(cb + r, nchans - 1u);
+  const uint32_t d = min(dw + idx % MW, ndms - 1u);
+  const int32_t *row = table + (uint64_t)c * ndms;
+  const int32_t base = row[d0];
+  const uint32_t rel = (uint32_t)(row[d] - base);
+  uint32_t w;
+  if (PACK == 0) {
+    w = r * rw + rel;
+  } else {
+    const uint32_t q = rel + ((uint32_t)base & (PACK == 1 ? 1u : 3u));
+    w = r * rw + (q >> 1) + (q & 1u) * (rw >> 1);
+  }
+  return (int32_t)(4u * w);
+}
+
+// The offsets do not depend on the time tile, so they are tabulated once per
+// plan and tile shape (trial_offset_kernel) as [DM tile][channel block][wave]
+// [register][16] and each block's registers are one load per register from a
+// wave-uniform base plus the lane's constant (lane % 16) word.
 template <int MW, int CB, int PACK>
 struct RelOffsets {
   static constexpr int N = (CB * MW + 15) / 16;
+  static constexpr int WORDS = 16 * N;  // table words per (DM tile, block, wave)
   int32_t v[N];
-  __device__ __forceinline__ void load(const int32_t *__restrict__ table, uint32_t ndms,
-                                       uint32_t nchans, uint32_t cb, uint32_t d0, uint32_t dw,
-                                       uint32_t lane, uint32_t rw) {
+  __device__ __forceinline__ void load(const int32_t *__restrict__ p, uint32_t lane) {
 #pragma unroll
-    for (int n = 0; n < N; ++n) {
-      const uint32_t idx = 16u * n + (lane & 15u);
-      const uint32_t r = idx / MW;
-      const uint32_t c = min(cb + r, nchans - 1u);
-      const uint32_t d = min(dw + idx % MW, ndms - 1u);
-      const int32_t *row = table + (uint64_t)c * ndms;
-      const int32_t base = row[d0];
-      const uint32_t rel = (uint32_t)(row[d] - base);
-      uint32_t w;
-      if (PACK == 0) {
-        w = r * rw + rel;
-      } else {
-        const uint32_t q = rel + ((uint32_t)base & (PACK == 1 ? 1u : 3u));
-        w = r * rw + (q >> 1) + (q & 1u) * (rw >> 1);
-      }
-      v[n] = (int32_t)(4u * w);
-    }
+    for (int n = 0; n < N; ++n) v[n] = p[16u * n + (lane & 15u)];
   }
   // offset of (r, i) in every lane: DPP row_newbcast of lane idx % 16 (the
   // callers' loops are unrolled, so the switch folds to one case)
@@ -147,10 +154,13 @@ struct F32Tile {
   static uint32_t lds_words_per_row(uint32_t rw) { return 2u * rw; }
   static bool fits(uint32_t) { return true; }
 
+  static constexpr int PACK = 0;
+  using Rel = RelOffsets<MW, CB_, PACK>;
   __device__ __forceinline__ static void run(const float *__restrict__ in, uint64_t ld,
-                             const int32_t *__restrict__ table, uint32_t nchans, uint32_t nsamp,
-                             uint32_t ndms, uint32_t rw, uint32_t cbr, float *__restrict__ out,
-                             uint32_t tt, uint32_t dt, float *lds) {
+                             const int32_t *__restrict__ table, const int32_t *__restrict__ reltab,
+                             uint32_t nchans, uint32_t nsamp, uint32_t ndms, uint32_t rw,
+                             uint32_t cbr, float *__restrict__ out, uint32_t tt, uint32_t dt,
+                             float *lds) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = uniform(threadIdx.x >> 6);
     const uint32_t t0 = tt * TT, d0 = dt * TD;
@@ -188,9 +198,11 @@ struct F32Tile {
 #pragma unroll
       for (int j = 0; j < K; ++j) acc[i][j] = 0.0f;
 
-    RelOffsets<MW, CB, 0> rel, rel_next;
+    Rel rel, rel_next;
+    const uint32_t nblocks = (nchans + cbr - 1u) / cbr;
+    const int32_t *rt = reltab + ((uint64_t)dt * nblocks * 4u + wave) * Rel::WORDS;
     stage(0, lds);
-    rel.load(table, ndms, nchans, 0, d0, dw, lane, rw);
+    rel.load(rt, lane);
     wait_vmem();
     __syncthreads();
     uint32_t k = 0;
@@ -199,7 +211,7 @@ struct F32Tile {
       const bool more = cb + cbr < nchans;
       if (more) {
         stage(cb + cbr, lds + ((k + 1u) & 1u) * bufw);
-        rel_next.load(table, ndms, nchans, cb + cbr, d0, dw, lane, rw);
+        rel_next.load(rt + (uint64_t)(k + 1u) * 4u * Rel::WORDS, lane);
       }
       const uint32_t nc = min(cbr, nchans - cb);
 #pragma unroll
@@ -288,8 +300,11 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
   static uint32_t row_words(uint32_t spread) { return G::row_words(spread); }
   static uint32_t lds_words_per_row(uint32_t rw) { return 2u * rw + raw_words(rw); }
 
+  static constexpr int PACK = IS_U8 ? 2 : 1;
+  using Rel = RelOffsets<MW, CB_, PACK>;
   __device__ __forceinline__ static void run(const T *__restrict__ in, uint64_t ld,
-                                             const int32_t *__restrict__ table, uint32_t nchans,
+                                             const int32_t *__restrict__ table,
+                                             const int32_t *__restrict__ reltab, uint32_t nchans,
                                              uint32_t nsamp, uint32_t ndms, uint32_t rw,
                                              uint32_t cbr, float *__restrict__ out, uint32_t tt,
                                              uint32_t dt, float *ldsf) {
@@ -382,9 +397,11 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
     asm("s_mov_b32 %0, 0x3f800000" : "=s"(one_bits));
     const float one = __builtin_bit_cast(float, one_bits);
 
-    RelOffsets<MW, CB, IS_U8 ? 2 : 1> rel, rel_next;
+    Rel rel, rel_next;
+    const uint32_t nblocks = (nchans + cbr - 1u) / cbr;
+    const int32_t *rt = reltab + ((uint64_t)dt * nblocks * 4u + wave) * Rel::WORDS;
     stage(0, lds);
-    rel.load(table, ndms, nchans, 0, d0, dw, lane, rw);
+    rel.load(rt, lane);
     wait_vmem();
     __syncthreads();
     build(0, lds);
@@ -395,7 +412,7 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
       const bool more = cb + cbr < nchans;
       if (more) {
         stage(cb + cbr, lds + ((k + 1u) & 1u) * bufw);
-        rel_next.load(table, ndms, nchans, cb + cbr, d0, dw, lane, rw);
+        rel_next.load(rt + (uint64_t)(k + 1u) * 4u * Rel::WORDS, lane);
       }
       const uint32_t nc = min(cbr, nchans - cb);
       if (IS_U8) {
@@ -499,6 +516,7 @@ template <typename Tile>
 __global__ __launch_bounds__(256) void dedisp_kernel(const typename Tile::T *__restrict__ in,
                                                      uint64_t ld,
                                                      const int32_t *__restrict__ table,
+                                                     const int32_t *__restrict__ reltab,
                                                      uint32_t nchans, uint32_t nsamp,
                                                      uint32_t ndms, uint32_t rw, uint32_t cbr,
                                                      uint32_t n_dtiles, uint32_t n_ttiles,
@@ -520,7 +538,29 @@ __global__ __launch_bounds__(256) void dedisp_kernel(const typename Tile::T *__r
     tt = b / n_dtiles;
     dt = b % n_dtiles;
   }
-  Tile::run(in, ld, table, nchans, nsamp, ndms, rw, cbr, out, tt, dt, lds);
+  Tile::run(in, ld, table, reltab, nchans, nsamp, ndms, rw, cbr, out, tt, dt, lds);
+}
+
+// Trial-offset table of one tile shape: entry ((dt * nblocks + k) * 4 + w)
+// * WORDS + 16 n + l holds the byte offset of (r, i) = divmod(16 n + l, MW)
+// for channel block k (cbr channels) and wave w of DM tile dt.
+template <typename Tile>
+__global__ __launch_bounds__(256) void trial_offset_kernel(const int32_t *__restrict__ table,
+                                                           uint32_t nchans, uint32_t ndms,
+                                                           uint32_t rw, uint32_t cbr,
+                                                           uint32_t nblocks, uint64_t total,
+                                                           int32_t *__restrict__ reltab) {
+  using Rel = typename Tile::Rel;
+  constexpr uint32_t MW = Tile::TD / 4;
+  const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (e >= total) return;
+  const uint32_t idx = (uint32_t)(e % Rel::WORDS);
+  const uint64_t g = e / Rel::WORDS;  // (dt * nblocks + k) * 4 + w
+  const uint32_t w = (uint32_t)(g & 3u);
+  const uint64_t blk = g >> 2;
+  const uint32_t k = (uint32_t)(blk % nblocks), dt = (uint32_t)(blk / nblocks);
+  const uint32_t d0 = dt * Tile::TD;
+  reltab[e] = rel_entry<MW, Tile::PACK>(table, ndms, nchans, k * cbr, d0, d0 + w * MW, idx, rw);
 }
 
 // Multi-beam item numbering (both launch shapes): a beam's items are
@@ -559,7 +599,8 @@ __device__ __forceinline__ void run_item(const BeamDesc *__restrict__ beams, con
   const BeamDesc &bd = *it.bd;
   char *base = reinterpret_cast<char *>(const_cast<BeamDesc *>(beams));
   Tile::run(reinterpret_cast<const typename Tile::T *>(base + bd.in_off), bd.ld,
-            reinterpret_cast<const int32_t *>(base + bd.table_off), bd.nchans, bd.nsamp, bd.ndms,
+            reinterpret_cast<const int32_t *>(base + bd.table_off),
+            reinterpret_cast<const int32_t *>(base + bd.rel_off), bd.nchans, bd.nsamp, bd.ndms,
             bd.row_words, cbr, reinterpret_cast<float *>(base + bd.out_off), it.tt, it.dt, lds);
 }
 
@@ -613,7 +654,12 @@ template <> struct VariantDef<5> { using F32 = F32Tile<32, 2, 2>;  using F16 = F
 template <> struct VariantDef<6> { using F32 = F32Tile<16, 4, 4, 8>; using F16 = F16Tile<16, 2, 4, 16>; using U8 = U8Tile<16, 2, 4, 16>; };
 template <> struct VariantDef<7> { using F32 = F32Tile<8, 4, 8, 8>;  using F16 = F16Tile<8, 2, 8, 8>;   using U8 = U8Tile<8, 2, 8, 8>; };
 template <> struct VariantDef<8> { using F32 = F32Tile<16, 4, 4, 4>; using F16 = F16Tile<16, 2, 4, 8>;  using U8 = U8Tile<16, 2, 4, 8>; };
-constexpr int kNumVariants = 8;
+// few trials per wave, many samples per lane: one address add per 8 (F32)
+// or 16 (packed) adds, a smaller delay spread per tile, larger windows
+template <> struct VariantDef<9> { using F32 = F32Tile<8, 8, 4>;    using F16 = F16Tile<8, 4, 4>;   using U8 = U8Tile<8, 4, 4>; };
+template <> struct VariantDef<10> { using F32 = F32Tile<4, 16, 8>;  using F16 = F16Tile<4, 8, 8>;   using U8 = U8Tile<4, 8, 8>; };
+template <> struct VariantDef<11> { using F32 = F32Tile<4, 16, 4>;  using F16 = F16Tile<4, 8, 4>;   using U8 = U8Tile<4, 8, 4>; };
+constexpr int kNumVariants = 11;
 
 template <typename V, typename Fn>
 int pick_dtype(int dtype, Fn &&fn) {
@@ -637,6 +683,9 @@ int with_tile(int dtype, int variant, Fn &&fn) {
     case 6: return pick_dtype<VariantDef<6>>(dtype, fn);
     case 7: return pick_dtype<VariantDef<7>>(dtype, fn);
     case 8: return pick_dtype<VariantDef<8>>(dtype, fn);
+    case 9: return pick_dtype<VariantDef<9>>(dtype, fn);
+    case 10: return pick_dtype<VariantDef<10>>(dtype, fn);
+    case 11: return pick_dtype<VariantDef<11>>(dtype, fn);
   }
   set_error("unknown dedispersion variant %d", variant);
   return MDSM_ERR_ARG;
@@ -656,6 +705,8 @@ uint32_t stage_channels(uint32_t rw) {
 }  // namespace
 
 int num_variants(int) { return kNumVariants; }
+
+int preferred_variant(int dtype) { return dtype == MDSM_F32 ? 8 : 2; }
 
 TileShape variant_shape(int dtype, int variant) {
   TileShape sh{0, 0, 0};
@@ -683,11 +734,50 @@ uint32_t input_alignment(int dtype) {
   return dtype == MDSM_F32 ? 1u : (dtype == MDSM_F16 ? 2u : 4u);
 }
 
+uint32_t stage_channels(int dtype, int variant, uint32_t rw) {
+  uint32_t cbr = 0;
+  with_tile(dtype, variant, [&](auto *tp) {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    cbr = rw ? stage_channels<Tile>(rw) : 0u;
+    return MDSM_OK;
+  });
+  return cbr;
+}
+
+uint64_t trial_offset_words(int dtype, int variant, uint32_t nchans, uint32_t ndms, uint32_t cbr) {
+  uint64_t n = 0;
+  with_tile(dtype, variant, [&](auto *tp) {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    const uint64_t n_dtiles = (ndms + Tile::TD - 1) / Tile::TD;
+    const uint64_t nblocks = cbr ? (nchans + cbr - 1) / cbr : 0;
+    n = n_dtiles * nblocks * 4u * Tile::Rel::WORDS;
+    return MDSM_OK;
+  });
+  return n;
+}
+
+int launch_trial_offsets(int dtype, int variant, const int32_t *table, uint32_t nchans,
+                         uint32_t ndms, uint32_t rw, uint32_t cbr, int32_t *reltab, hipStream_t s) {
+  MDSM_REQUIRE(rw > 0 && cbr > 0, MDSM_ERR_ARG, "tile variant %d cannot stage this survey", variant);
+  return with_tile(dtype, variant, [&](auto *tp) -> int {
+    using Tile = std::remove_pointer_t<decltype(tp)>;
+    const uint32_t nblocks = (nchans + cbr - 1) / cbr;
+    const uint64_t total = trial_offset_words(dtype, variant, nchans, ndms, cbr);
+    const uint64_t grid = (total + 255u) / 256u;
+    MDSM_REQUIRE(grid < (1ull << 31), MDSM_ERR_ARG, "trial offset table too large");
+    hipLaunchKernelGGL(trial_offset_kernel<Tile>, dim3((uint32_t)grid), dim3(256), 0, s, table,
+                       nchans, ndms, rw, cbr, nblocks, total, reltab);
+    MDSM_HIP(hipGetLastError());
+    return MDSM_OK;
+  });
+}
+
 int launch_dedisperse(int dtype, int variant, const void *in, uint64_t ld, const int32_t *table,
-                      uint32_t nchans, uint32_t nsamp, uint32_t ndms, uint32_t rw, float *out,
-                      hipStream_t s) {
+                      const int32_t *reltab, uint32_t nchans, uint32_t nsamp, uint32_t ndms,
+                      uint32_t rw, float *out, hipStream_t s) {
   MDSM_REQUIRE(rw > 0, MDSM_ERR_ARG,
                "delay spread too large for tile variant %d (choose another variant)", variant);
+  MDSM_REQUIRE(reltab, MDSM_ERR_STATE, "trial offset table missing for tile variant %d", variant);
   return with_tile(dtype, variant, [&](auto *tp) -> int {
     using Tile = std::remove_pointer_t<decltype(tp)>;
     const uint32_t n_ttiles = (nsamp + Tile::TT - 1) / Tile::TT;
@@ -700,8 +790,8 @@ int launch_dedisperse(int dtype, int variant, const void *in, uint64_t ld, const
     MDSM_REQUIRE(cbr >= 1, MDSM_ERR_ARG, "delay spread too large for LDS staging (%u words)", rw);
     const size_t lds = (size_t)cbr * Tile::lds_words_per_row(rw) * sizeof(float);
     hipLaunchKernelGGL(dedisp_kernel<Tile>, dim3((uint32_t)blocks), dim3(256), lds, s,
-                       reinterpret_cast<const typename Tile::T *>(in), ld, table, nchans, nsamp,
-                       ndms, rw, cbr, n_dtiles, n_ttiles, remap, out);
+                       reinterpret_cast<const typename Tile::T *>(in), ld, table, reltab, nchans,
+                       nsamp, ndms, rw, cbr, n_dtiles, n_ttiles, remap, out);
     MDSM_HIP(hipGetLastError());
     return MDSM_OK;
   });

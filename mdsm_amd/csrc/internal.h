@@ -60,9 +60,20 @@ uint32_t input_alignment(int dtype);
 int build_delay_table(const mdsm_survey_t *s, std::vector<int32_t> &table, uint32_t *maxshift);
 
 // ---------------------------------------------------------------- kernels
+// Untuned default tile variant per storage type (the fastest measured on a
+// 1024-channel, 4096-trial beam).
+int preferred_variant(int dtype);
+// Channels staged per LDS block for a tile variant and row width (0: does not fit).
+uint32_t stage_channels(int dtype, int variant, uint32_t row_words);
+// Words of a variant's trial-offset table, and the kernel that fills it
+// from the delay table (the table depends on the row width and the block).
+uint64_t trial_offset_words(int dtype, int variant, uint32_t nchans, uint32_t ndms, uint32_t cbr);
+int launch_trial_offsets(int dtype, int variant, const int32_t *table, uint32_t nchans,
+                         uint32_t ndms, uint32_t row_words, uint32_t cbr, int32_t *reltab,
+                         hipStream_t stream);
 int launch_dedisperse(int dtype, int variant, const void *in, uint64_t ld, const int32_t *table,
-                      uint32_t nchans, uint32_t nsamp, uint32_t ndms, uint32_t row_words,
-                      float *out, hipStream_t stream);
+                      const int32_t *reltab, uint32_t nchans, uint32_t nsamp, uint32_t ndms,
+                      uint32_t row_words, float *out, hipStream_t stream);
 
 // Per-beam descriptor of the persistent batched dedisperser.  Buffers are
 // given as byte offsets from the descriptor array itself: pointers derived
@@ -72,6 +83,7 @@ struct BeamDesc {
   int64_t in_off;
   int64_t out_off;
   int64_t table_off;
+  int64_t rel_off;  // trial-offset table of the bound variant
   uint64_t ld;
   uint32_t nchans, nsamp, ndms, row_words;
   uint32_t n_ttiles, n_dtiles;
