@@ -343,12 +343,15 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
       }
     };
     // build the pair copies of rows [cb, cb+nc) in buffer b
+    // (F16: wave w builds rows w, w+4, ..., two Q words per lane from one
+    // 8-byte and one 4-byte LDS read; Q[wp-1], which no trial reads, may
+    // take the stale word past P)
     auto build = [&](uint32_t cb, uint32_t *b) {
       const uint32_t nc = min(cbr, nchans - cb);
-      for (uint32_t r = 0; r < nc; ++r) {
-        uint32_t *P = b + r * rw;
-        uint32_t *Q = P + wp;
-        if (IS_U8) {
+      if (IS_U8) {
+        for (uint32_t r = 0; r < nc; ++r) {
+          uint32_t *P = b + r * rw;
+          uint32_t *Q = P + wp;
           const uint32_t *R = raw + r * rawn;
           for (uint32_t n = tid; 2u * n < wp; n += 256u) {
             const uint32_t x = R[n], y = R[n + 1];
@@ -357,9 +360,17 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
             Q[2 * n] = __builtin_amdgcn_perm(y, x, 0x0C020C01u);
             Q[2 * n + 1] = __builtin_amdgcn_perm(y, x, 0x0C040C03u);
           }
-        } else {
-          for (uint32_t w = tid; w + 1u < wp; w += 256u)
-            Q[w] = __builtin_amdgcn_alignbit(P[w + 1], P[w], 16);
+        }
+      } else {
+        for (uint32_t r = wave; r < nc; r += 4u) {
+          uint32_t *P = b + r * rw;
+          uint32_t *Q = P + wp;
+          for (uint32_t w = 2u * lane; w + 1u < wp; w += 128u) {
+            const uint2 a = *reinterpret_cast<const uint2 *>(P + w);
+            const uint32_t c = P[w + 2];
+            *reinterpret_cast<uint2 *>(Q + w) = make_uint2(__builtin_amdgcn_alignbit(a.y, a.x, 16),
+                                                           __builtin_amdgcn_alignbit(c, a.y, 16));
+          }
         }
       }
     };
@@ -659,7 +670,11 @@ template <> struct VariantDef<8> { using F32 = F32Tile<16, 4, 4, 4>; using F16 =
 template <> struct VariantDef<9> { using F32 = F32Tile<8, 8, 4>;    using F16 = F16Tile<8, 4, 4>;   using U8 = U8Tile<8, 4, 4>; };
 template <> struct VariantDef<10> { using F32 = F32Tile<4, 16, 8>;  using F16 = F16Tile<4, 8, 8>;   using U8 = U8Tile<4, 8, 8>; };
 template <> struct VariantDef<11> { using F32 = F32Tile<4, 16, 4>;  using F16 = F16Tile<4, 8, 4>;   using U8 = U8Tile<4, 8, 4>; };
-constexpr int kNumVariants = 11;
+// more samples per lane at 12 or 16 trials per wave: one address add per
+// 8 or 6 packed adds
+template <> struct VariantDef<12> { using F32 = F32Tile<12, 6, 4>;  using F16 = F16Tile<12, 4, 4>;  using U8 = U8Tile<12, 4, 4>; };
+template <> struct VariantDef<13> { using F32 = F32Tile<16, 6, 4>;  using F16 = F16Tile<16, 3, 4>;  using U8 = U8Tile<16, 3, 4>; };
+constexpr int kNumVariants = 13;
 
 template <typename V, typename Fn>
 int pick_dtype(int dtype, Fn &&fn) {
@@ -686,6 +701,8 @@ int with_tile(int dtype, int variant, Fn &&fn) {
     case 9: return pick_dtype<VariantDef<9>>(dtype, fn);
     case 10: return pick_dtype<VariantDef<10>>(dtype, fn);
     case 11: return pick_dtype<VariantDef<11>>(dtype, fn);
+    case 12: return pick_dtype<VariantDef<12>>(dtype, fn);
+    case 13: return pick_dtype<VariantDef<13>>(dtype, fn);
   }
   set_error("unknown dedispersion variant %d", variant);
   return MDSM_ERR_ARG;
