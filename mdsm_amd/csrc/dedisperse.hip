@@ -345,20 +345,22 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
     // build the pair copies of rows [cb, cb+nc) in buffer b
     // (F16: wave w builds rows w, w+4, ..., two Q words per lane from one
     // 8-byte and one 4-byte LDS read; Q[wp-1], which no trial reads, may
-    // take the stale word past P)
+    // take the stale word past P.  U8: also one wave per row.)
     auto build = [&](uint32_t cb, uint32_t *b) {
       const uint32_t nc = min(cbr, nchans - cb);
       if (IS_U8) {
-        for (uint32_t r = 0; r < nc; ++r) {
+        // wave w widens rows w, w+4, ...: lane n turns raw dwords n, n+1
+        // into P words 2n, 2n+1 and Q words 2n, 2n+1 (two 8-byte writes)
+        for (uint32_t r = wave; r < nc; r += 4u) {
           uint32_t *P = b + r * rw;
           uint32_t *Q = P + wp;
           const uint32_t *R = raw + r * rawn;
-          for (uint32_t n = tid; 2u * n < wp; n += 256u) {
+          for (uint32_t n = lane; 2u * n < wp; n += 64u) {
             const uint32_t x = R[n], y = R[n + 1];
-            P[2 * n] = __builtin_amdgcn_perm(y, x, 0x0C010C00u);
-            P[2 * n + 1] = __builtin_amdgcn_perm(y, x, 0x0C030C02u);
-            Q[2 * n] = __builtin_amdgcn_perm(y, x, 0x0C020C01u);
-            Q[2 * n + 1] = __builtin_amdgcn_perm(y, x, 0x0C040C03u);
+            *reinterpret_cast<uint2 *>(P + 2 * n) =
+                make_uint2(__builtin_amdgcn_perm(y, x, 0x0C010C00u), __builtin_amdgcn_perm(y, x, 0x0C030C02u));
+            *reinterpret_cast<uint2 *>(Q + 2 * n) =
+                make_uint2(__builtin_amdgcn_perm(y, x, 0x0C020C01u), __builtin_amdgcn_perm(y, x, 0x0C040C03u));
           }
         }
       } else {
