@@ -269,6 +269,17 @@ def main():
     total_adds = sum(s.ndms * s.nsamp * s.nchans for s in surveys)
     log(f"[batch] {len(beams)} beams, sum M*N = {total_work:.3e}, adds = {total_adds:.3e}")
 
+    big = max(range(len(beams)), key=lambda i: surveys[i].nchans * surveys[i].ndms)
+
+    def tune_variant(dtype, ins):
+        """Autotuned tile shape on the largest beam truncated to 2^17 samples."""
+        sb = surveys[big]
+        probe = abi.Survey.make(sb.fch1, sb.foff, sb.nchans, sb.tsamp, sb.lowdm, sb.dmstep,
+                                sb.ndms, 1 << 17)
+        pp = abi.Plan(probe, dtype)
+        v, _ = pp.autotune(ins[big][0].ptr, ins[big][1], outs[big].ptr, stream)
+        return v
+
     def run_batch(dtype, tag):
         batch = abi.Batch(surveys, dtype)
         isz = {abi.F32: 4, abi.F16: 2, abi.U8: 1}[dtype]
@@ -281,13 +292,7 @@ def main():
             ins.append((d, ld))
         st.free()
         abi.synchronize()
-        # autotune the tile shape on the largest beam truncated to 2^17 samples
-        big = max(range(len(beams)), key=lambda i: surveys[i].nchans * surveys[i].ndms)
-        sb = surveys[big]
-        probe = abi.Survey.make(sb.fch1, sb.foff, sb.nchans, sb.tsamp, sb.lowdm, sb.dmstep,
-                                sb.ndms, 1 << 17)
-        pp = abi.Plan(probe, dtype)
-        v, _ = pp.autotune(ins[big][0].ptr, ins[big][1], outs[big].ptr, stream)
+        v = tune_variant(dtype, ins)
         batch.set_variant(v)
         # launch shape: one timed launch of each, keep the faster
         mode_ms = {}
@@ -363,17 +368,18 @@ def main():
             ins32.append((d, ld))
         st.free()
         abi.synchronize()
+        v32 = tune_variant(abi.F32, ins32)  # its own best tile shape, tuned like the batch's
         for p in plans:
-            p.set_variant(v16 if v16 else 0)
+            p.set_variant(v32)
 
         def plain_step():
             for b, p in enumerate(plans):
                 p.dedisperse(ins32[b][0].ptr, ins32[b][1], outs[b].ptr, stream)
         ms32 = time_steps(plain_step, args.steps, args.warmup, stream)
         B32 = sum(bytes_per_beam(s, plans[i].maxshift, 4) for i, s in enumerate(surveys))
-        plain = {"ms": ms32, "dm_samples_per_s": total_work / ms32 * 1e3,
+        plain = {"ms": ms32, "dm_samples_per_s": total_work / ms32 * 1e3, "tile_variant": v32,
                  "hbm_frac": B32 / (ms32 / 1e3) / HBM_PEAK, "launches_per_step": len(plans)}
-        log(f"[plain fp32] {ms32:.1f} ms/step, {total_work / ms32 * 1e3:.3e} DM*samples/s")
+        log(f"[plain fp32] variant {v32}: {ms32:.1f} ms/step, {total_work / ms32 * 1e3:.3e} DM*samples/s")
         for d, _ in ins32:
             d.free()
 

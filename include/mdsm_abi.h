@@ -117,7 +117,10 @@ int mdsm_plan_destroy(mdsm_plan plan);
 int mdsm_plan_info(mdsm_plan plan, uint32_t *maxshift, uint64_t *min_ld);
 /* Device pointer to the plan's int32[nchans][ndms] delay table. */
 int mdsm_plan_device_table(mdsm_plan plan, const int32_t **table);
-/* Kernel tile variant (0 = autotuned default); see mdsm_tile_variants(). */
+/* Kernel tile variant (0 = the storage type's default); see
+ * mdsm_tile_variants().  Setting a variant builds its trial-offset table
+ * (one allocation and one kernel, synchronous); a variant first used by
+ * mdsm_dedisperse without being set builds it there, once. */
 int mdsm_plan_set_variant(mdsm_plan plan, int variant);
 int mdsm_plan_get_variant(mdsm_plan plan, int *variant);
 int mdsm_tile_variants(int in_dtype, int *count);
@@ -215,9 +218,9 @@ int mdsm_classify_cluster_ref(const double *time, const double *dm, const double
                               float fch1, float foff, uint32_t nchans, float *score, double *info);
 
 /* ---------------------------------------------------------- multi-beam batch
- * Many beams of uneven nchans/nsamp/ndms dedispersed by one persistent
- * launch: the workgroups pull (beam, tile) items from a device work queue,
- * largest beams first.  Replaces the per-beam host threads of
+ * Many beams of uneven nchans/nsamp/ndms dedispersed by one launch over
+ * all (beam, tile) items, largest beams first (launch shapes below).
+ * Replaces the per-beam host threads of
  * multibeam_dedispersion_thread.cu:464-839 (one launch per beam per chunk). */
 typedef struct mdsm_batch_s *mdsm_batch;
 int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, int in_dtype,
@@ -225,26 +228,28 @@ int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, int in_dtype,
 int mdsm_batch_destroy(mdsm_batch batch);
 int mdsm_batch_info(mdsm_batch batch, uint32_t beam, uint32_t *maxshift, uint64_t *min_ld);
 /* d_in[b]: beam b input [nchans_b][ld[b]]; d_out[b]: [ndms_b][nsamp_b].
- * bind uploads the beam descriptors (synchronous); launch enqueues only a
- * counter clear and the persistent kernel (no allocation, no host copy), so
- * it may be captured into a hipGraph and replayed while the bound buffers
- * stay valid.  dedisperse = bind + launch. */
+ * bind uploads the beam descriptors and builds missing trial-offset tables
+ * (synchronous); launch enqueues only the dedispersion kernel (and, in the
+ * persistent shape, a counter clear) with no allocation and no host copy,
+ * so it may be captured into a hipGraph and replayed while the bound
+ * buffers stay valid.  dedisperse = bind + launch. */
 int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                     float *const *d_out);
 int mdsm_batch_launch(mdsm_batch batch, void *stream);
 int mdsm_batch_dedisperse(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                           float *const *d_out, void *stream);
 int mdsm_batch_set_variant(mdsm_batch batch, int variant);
-/* Launch shape (takes effect at the next bind): PERSISTENT (default) runs
- * resident workgroups that drain a device queue of tiles; FLAT launches one
+/* Launch shape (takes effect at the next bind): FLAT (default) launches one
  * workgroup per tile, numbered so that the DM tiles re-reading one input
- * window share an XCD (and its L2).  Both are one launch per step. */
+ * window share an XCD (and its L2); PERSISTENT runs resident workgroups that
+ * drain eight device queues of tiles (one per XCD) in the same order.  Both
+ * are one launch per step. */
 enum { MDSM_BATCH_PERSISTENT = 0, MDSM_BATCH_FLAT = 1 };
 int mdsm_batch_set_mode(mdsm_batch batch, int mode);
 
 /* ------------------------------------------------------- search pipeline
  * The per-chunk sequence of a multi-beam search: for every beam bandpass
- * normalisation of the raw FP32 input into the storage type, one persistent
+ * normalisation of the raw FP32 input into the storage type, one
  * batched dedispersion, then per-block statistics and boxcar detection into
  * a per-beam candidate list.  The reference runs these stages from separate
  * host threads (multibeam_dedispersion_thread.cu:464-839,

@@ -23,8 +23,8 @@ using namespace mdsm;
 struct mdsm_batch_s {
   int dtype = MDSM_F32;
   int variant = 0;
-  int mode = MDSM_BATCH_PERSISTENT;
-  uint32_t bound_mode = MDSM_BATCH_PERSISTENT;
+  int mode = MDSM_BATCH_FLAT;  // the faster shape measured on the 32-beam batch
+  uint32_t bound_mode = MDSM_BATCH_FLAT;
   std::vector<mdsm_survey_t> beams;
   std::vector<uint32_t> maxshift;
   std::vector<std::vector<int32_t>> h_tables;

@@ -9,9 +9,10 @@
 // t0 + TT + D[c][d_last]) are staged into one of two LDS buffers while the
 // waves add the previous block, so HBM/L2 latency overlaps the adds.  Each
 // staged sample is reused by TD trials.  The per-trial offsets inside a
-// window are wave-uniform; a block's offsets sit one per lane in a VGPR and
-// are moved to an SGPR with v_readlane, so one trial costs one address add
-// for K LDS reads (issued in pairs as ds_read2st64).  The adds are bound by
+// window are wave-uniform and independent of the time tile: they are
+// tabulated once per plan, a block's offsets sit 16 per VGPR, and one trial
+// costs one DPP-broadcast address add for K LDS reads (issued in pairs as
+// ds_read2st64).  The adds are bound by
 // LDS read bandwidth (~128 B/clk/CU), not by HBM, so the storage formats
 // trade LDS bytes per add:
 //   F32  4 B per add: one float per LDS word, staged by direct buffer->LDS
@@ -20,8 +21,9 @@
 //        copies (even and odd alignment), so any wave-uniform offset reads
 //        an aligned word; v_fma_mix_f32 (half operand selected by op_sel,
 //        times 1.0, plus the FP32 accumulator) adds one half with one IEEE
-//        round-to-nearest-even.  (v_dot2c_f32_f16 would also take one
-//        instruction but rounds exact ties differently from IEEE.)
+//        round-to-nearest-even.  (v_dot2_f32_f16 would add two halves per
+//        instruction but is not correctly rounded: it drops low bits of
+//        the exact sum in cancellation and wide-exponent cases.)
 //   U8   2 B per add: bytes widened to pairs of u16 in the same two-copy
 //        layout, summed exactly with v_pk_add_u16 (two adds per
 //        instruction) and folded into u32 every <= 257 channels.
