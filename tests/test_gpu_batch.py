@@ -54,7 +54,7 @@ def test_batch_uneven_beams_bitexact(dtype):
         outs.append(do)
     for mode in (abi.BATCH_PERSISTENT, abi.BATCH_FLAT):
         batch.set_mode(mode)
-        for v in (0, 1, 4):
+        for v in (0, 1, 4, 9, 12, 13):
             for b, s in enumerate(beams):  # every tile must be written again
                 outs[b].upload(np.full(s.ndms * s.nsamp, np.nan, np.float32))
             print(f"[batch dtype {dtype}] mode {mode} variant {v}", flush=True, file=sys.stderr)

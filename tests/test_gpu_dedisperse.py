@@ -99,6 +99,26 @@ def test_f16_u8_bitexact_and_error():
     assert rel < 2e-3
 
 
+@pytest.mark.parametrize("shape", [(37, 45, 1000, 2.5, 1.7), (130, 129, 777, 0.0, 9.0), (600, 70, 3001, 0.0, 0.3)])
+def test_packed_ragged_shapes_all_variants(shape):
+    """F16 and U8 storage, every tile variant, on shapes that leave partial
+    channel blocks, DM tiles and time tiles (odd channel counts for the pair
+    copies; 600 channels cross the U8 16-bit fold at 257)."""
+    C, M, N, lo, step = shape
+    s = _survey(N, nchans=C, ndms=M, lowdm=lo, dmstep=step, foff=-0.078125 * 256 / C / 4)
+    t = _table(s)
+    S = int(t.max())
+    _say(f"[packed ragged] C={C} M={M} N={N} S={S}")
+    rng = np.random.default_rng(C * 7 + M)
+    h = O.f32_to_f16(rng.standard_normal((C, N + S + 5)).astype(np.float32))
+    u = rng.integers(0, 256, (C, N + S + 5)).astype(np.uint8)
+    ref16 = O.dedisperse_f16(h, t, N)
+    ref8 = O.dedisperse_u8(u, t, N)
+    for v in range(1, abi.tile_variants() + 1):
+        assert np.array_equal(abi.dedisperse_host(s, h, abi.F16, v), ref16), ("f16", v)
+        assert np.array_equal(abi.dedisperse_host(s, u, abi.U8, v), ref8), ("u8", v)
+
+
 def test_reference_delay_mode():
     s = _survey(8192, mode=abi.DELAY_REFERENCE)
     t = O.delay_table_ref(s.fch1, s.foff, 256, s.tsamp, s.lowdm, s.dmstep, 256)
