@@ -245,7 +245,8 @@ struct mdsm_plan_s {
 };
 
 // Build the trial-offset table of variant v on stream s if it is missing
-// (one allocation and one kernel, the first time the variant is used).
+// (one allocation and one kernel, the first time the variant is used; the
+// stream is then synchronised so that later calls on any stream see it).
 static int ensure_trial_offsets(mdsm_plan_s *p, int v, hipStream_t s) {
   if (p->d_rel[v]) return MDSM_OK;
   const uint32_t rw = p->row_words[v];
@@ -259,7 +260,11 @@ static int ensure_trial_offsets(mdsm_plan_s *p, int v, hipStream_t s) {
     set_error("hipMalloc(trial offsets) failed: %s", hipGetErrorString(e));
     return MDSM_ERR_NOMEM;
   }
-  const int rc = launch_trial_offsets(p->dtype, v, p->d_table, p->s.nchans, p->s.ndms, rw, cbr, d, s);
+  int rc = launch_trial_offsets(p->dtype, v, p->d_table, p->s.nchans, p->s.ndms, rw, cbr, d, s);
+  if (rc == MDSM_OK && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("trial offset table build failed");
+    rc = MDSM_ERR_HIP;
+  }
   if (rc != MDSM_OK) {
     (void)hipFree(d);
     return rc;
@@ -362,7 +367,6 @@ extern "C" int mdsm_plan_set_variant(mdsm_plan plan, int variant) {
   if (plan->row_words[v] && stage_channels(plan->dtype, v, plan->row_words[v])) {
     const int rc = ensure_trial_offsets(plan, v, nullptr);
     if (rc != MDSM_OK) return rc;
-    MDSM_HIP(hipStreamSynchronize(nullptr));
   }
   return MDSM_OK;
 }
