@@ -6,10 +6,14 @@ Headline workload (one step = one pass of the dedisperser over all beams):
 fch1 1500 MHz over a 300 MHz band, tsamp 64 us), Gaussian noise with
 per-channel bandpass gain U(0.5, 2) and offset U(0, 10) plus 20 injected
 pulses per beam, bandpass-normalised to FP16 storage, dedispersed by one
-persistent launch with FP32 accumulation.  The same beams are also run
-through the plain path (FP32 storage, one launch per beam) for the
-batched/FP16 vs plain/FP32 ratio, and a single full beam (1024 channels,
-2^20 samples, 4096 DMs at step 0.2) through the plain FP32 path.
+launch with FP32 accumulation.  The same beams are also run with U8
+storage, through the plain path (FP32 storage, one launch per beam) for the
+batched/FP16 vs plain/FP32 ratio, and batched with FP32 storage; a single
+full beam (1024 channels, 2^20 samples, 4096 DMs at step 0.2) runs through
+the plain FP32 path.  For four beams, sampled DM rows of every storage type
+are compared with the oracle (skip with --no-precision): bit-exact against
+the same-storage restatement, error against the FP64 sum of the
+unquantised input, and the candidate lists of those rows.
 
 All inputs are generated on the device and are resident in HBM before any
 timed region; timing uses HIP events on the stream the kernels run on.
@@ -127,7 +131,7 @@ def time_steps(fn, steps, warmup, stream):
     return ms / steps
 
 
-def spot_check(beam, plan_like_table, d_in, ld, dtype, d_out, rows):
+def spot_check(beam, plan_like_table, d_in, ld, dtype, d_out, rows, want_error=False):
     """Bit-exact check of selected full-length DM rows against the oracle."""
     import oracle as O
     s = beam["survey"]
@@ -141,9 +145,15 @@ def spot_check(beam, plan_like_table, d_in, ld, dtype, d_out, rows):
     else:
         ref = O.dedisperse_u8(x, np.ascontiguousarray(t), s.nsamp)
     ok = True
-    for i, d in enumerate(rows):
-        got = d_out.download(np.float32, (s.nsamp,), offset=int(d) * s.nsamp * 4)
-        ok &= bool(np.array_equal(got, ref[i]))
+    got = np.stack([d_out.download(np.float32, (s.nsamp,), offset=int(d) * s.nsamp * 4) for d in rows])
+    for i in range(len(rows)):
+        ok &= bool(np.array_equal(got[i], ref[i]))
+    if dtype == abi.F32 and want_error:
+        # accumulation error of these rows against the FP64 sum of the same input
+        y64 = O.dedisperse_f64(x, np.ascontiguousarray(t), s.nsamp)
+        e = np.abs(got.astype(np.float64) - y64)
+        return ok, {"max_err_over_max": float(e.max() / np.abs(y64).max()),
+                    "max_err_over_rms": float(e.max() / np.sqrt((y64 * y64).mean()))}
     return ok
 
 
@@ -208,6 +218,79 @@ def cpu_baseline(target_s=12.0):
     return cpu, ref
 
 
+def precision_rows(beam):
+    """DM rows sampled for the precision report: first, last, and the rows of
+    the first two injected pulses."""
+    s = beam["survey"]
+    rows = [0, s.ndms - 1] + [min(s.ndms - 1, int(round(p.dm / s.dmstep))) for p in beam["pulses"][:2]]
+    return sorted(set(rows))
+
+
+def precision_report(beams, prec_rows, L=16384, thr=6.0, nwidths=7):
+    """Per beam and storage type, on the sampled rows: bit-exact against the
+    oracle of the same storage, max and rms error against the FP64 sum of the
+    unquantised normalised input (relative to the rms of that sum), and the
+    6-sigma candidate lists of the rows (GPU detect on the GPU rows against
+    the oracle's detect on the oracle's rows; also counted against the FP64
+    rows).  The raw input is regenerated on the host by the oracle's
+    generator, which the device generator matches bit for bit."""
+    import oracle as O
+    conv = {abi.F32: "f32", abi.F16: "f16", abi.U8: "u8"}
+    out = []
+    for b in sorted({b for _, b in prec_rows}):
+        beam = beams[b]
+        s = beam["survey"]
+        rng = np.random.default_rng(beam["seed"])
+        gain = rng.uniform(0.5, 2.0, s.nchans).astype(np.float32)
+        off = rng.uniform(0.0, 10.0, s.nchans).astype(np.float32)
+        lds = {dt: prec_rows[(dt, b)][1] for dt in conv if (dt, b) in prec_rows}
+        ldmax = max(lds.values())
+        t0 = time.perf_counter()
+        raw = O.generate(beam["seed"], s.nchans, s.fch1, s.foff, s.tsamp, 0, ldmax, gain, off,
+                         [(p.dm, p.start, p.width, p.amp) for p in beam["pulses"]], ldmax)
+        table, _ = abi.delay_table(s)
+        rec = {"beam": b, "nchans": s.nchans, "nsamp": s.nsamp, "ndms": s.ndms, "dmstep": s.dmstep}
+        for dt, ld in lds.items():
+            rows, _, got = prec_rows[(dt, b)]
+            tab = np.ascontiguousarray(table[:, rows])
+            m, sd = O.channel_stats(raw, ld)
+            x32 = O.normalise(raw, m, sd, "f32", ld)
+            y64 = O.dedisperse_f64(x32, tab, s.nsamp)
+            if dt == abi.F32:
+                ref = O.dedisperse_f32seq(x32, tab, s.nsamp)
+            elif dt == abi.F16:
+                ref = O.dedisperse_f16(O.normalise(raw, m, sd, "f16", ld), tab, s.nsamp)
+            else:
+                ref = O.dedisperse_u8(O.normalise(raw, m, sd, "u8", ld), tab, s.nsamp)
+            err = got.astype(np.float64) - y64
+            rms = float(np.sqrt((y64 * y64).mean()))
+            d_rows = abi.DeviceBuffer.from_numpy(np.ascontiguousarray(got))
+            nb = (s.nsamp + L - 1) // L
+            dm_, ds_ = abi.DeviceBuffer(len(rows) * nb * 8), abi.DeviceBuffer(len(rows) * nb * 8)
+            cap = 1 << 20
+            dc, dn = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+            abi.block_stats(d_rows.ptr, len(rows), s.nsamp, L, dm_.ptr, ds_.ptr)
+            abi.detect(d_rows.ptr, len(rows), s.nsamp, L, dm_.ptr, ds_.ptr, nwidths, thr, dc.ptr, cap, dn.ptr)
+            abi.synchronize()
+            c = abi.read_candidates(dc, dn, cap)
+            o_d, o_t, o_k, _, _ = O.detect(ref, L, nwidths, thr)
+            f_d, f_t, f_k, _, _ = O.detect(y64, L, nwidths, thr)
+            gpu_set = set(zip(c["dm_idx"].tolist(), c["sample"].tolist(), c["width_log2"].tolist()))
+            orc_set = set(zip(o_d.tolist(), o_t.tolist(), o_k.tolist()))
+            f64_set = set(zip(f_d.tolist(), f_t.tolist(), f_k.tolist()))
+            rec[conv[dt]] = {"rows": rows, "bitexact_vs_oracle": bool(np.array_equal(got, ref)),
+                             "max_err_over_max": float(np.abs(err).max() / np.abs(y64).max()),
+                             "max_err_over_rms": float(np.abs(err).max() / rms),
+                             "rms_err_over_rms": float(np.sqrt((err * err).mean()) / rms),
+                             "candidates": len(gpu_set), "candidates_equal_oracle": gpu_set == orc_set,
+                             "candidates_fp64_rows": len(f64_set),
+                             "candidates_differing_from_fp64": len(gpu_set ^ f64_set)}
+        rec["host_seconds"] = time.perf_counter() - t0
+        log(f"[precision] {json.dumps(rec)}")
+        out.append(rec)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +301,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-beam FP32 pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the batched FP16 launches")
+    ap.add_argument("--no-precision", dest="precision", action="store_false",
+                    help="skip the comparison of sampled DM rows of four beams with the oracle "
+                         "(FP64 error, bit-exact same-storage rows, candidate lists)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -225,6 +311,7 @@ def main():
         sys.exit(2)
     if args.profile:
         args.no_single = args.no_plain = args.no_cpu = True
+        args.precision = False
 
     stream_p = abi.C.c_void_p()
     abi.check(abi.lib().mdsm_stream_create(abi.C.byref(stream_p)))
@@ -250,14 +337,15 @@ def main():
         work = s.ndms * s.nsamp
         B = bytes_per_beam(s, plan.maxshift, 4)
         t, _ = abi.delay_table(s)
-        ok = spot_check(sb, t, d_in, ld, abi.F32, d_out, [0, 1, 2047, 4095])
+        ok, err64 = spot_check(sb, t, d_in, ld, abi.F32, d_out, [0, 1, 2047, 4095], want_error=True)
         ncand, ncl, found = detect_beam(sb, d_out, stream)
         log(f"[single] variant {v}: {ms:.2f} ms, {work / ms / 1e6:.3e} DM*samples/s, "
             f"spot check {'ok' if ok else 'FAILED'}, {ncand} cands, {ncl} clusters, {found}/20 pulses")
         result["single_fp32"] = {"ms": ms, "dm_samples_per_s": work / ms * 1e3, "variant": v,
                                  "hbm_frac": B / (ms / 1e3) / HBM_PEAK,
                                  "valu_frac": s.nchans * work / (ms / 1e3) / VALU_ADD_PEAK,
-                                 "spot_check_bitexact": ok, "candidates": ncand,
+                                 "spot_check_bitexact": ok, "error_vs_fp64_rows": err64,
+                                 "candidates": ncand,
                                  "clusters": ncl, "pulses_recovered": f"{found}/20"}
         del d_in, d_out, plan
 
@@ -270,6 +358,8 @@ def main():
     log(f"[batch] {len(beams)} beams, sum M*N = {total_work:.3e}, adds = {total_adds:.3e}")
 
     big = max(range(len(beams)), key=lambda i: surveys[i].nchans * surveys[i].ndms)
+    by_adds = sorted(range(len(beams)), key=lambda i: surveys[i].nchans * surveys[i].ndms * surveys[i].nsamp)
+    prec_beams = sorted({by_adds[-1], by_adds[0], by_adds[len(by_adds) // 3], by_adds[2 * len(by_adds) // 3]})
 
     def tune_variant(dtype, ins):
         """Autotuned tile shape on the largest beam truncated to 2^17 samples."""
@@ -280,19 +370,31 @@ def main():
         v, _ = pp.autotune(ins[big][0].ptr, ins[big][1], outs[big].ptr, stream)
         return v
 
-    def run_batch(dtype, tag):
+    prec_rows = {}  # (dtype, beam) -> (rows, ld, GPU output rows), with --precision
+
+    def capture_rows(dtype, ins):
+        for b in prec_beams:
+            s = surveys[b]
+            rows = precision_rows(beams[b])
+            got = np.stack([outs[b].download(np.float32, (s.nsamp,), offset=int(d) * s.nsamp * 4)
+                            for d in rows])
+            prec_rows[(dtype, b)] = (rows, ins[b][1], got)
+
+    def run_batch(dtype, tag, ins=None, v=None):
         batch = abi.Batch(surveys, dtype)
         isz = {abi.F32: 4, abi.F16: 2, abi.U8: 1}[dtype]
-        st = Stage(stream)
-        ins = []
-        for b, beam in enumerate(beams):
-            ld = batch.min_ld[b]
-            d = abi.DeviceBuffer(beam["survey"].nchans * ld * isz)
-            st.fill(beam, d.ptr, ld, dtype)
-            ins.append((d, ld))
-        st.free()
-        abi.synchronize()
-        v = tune_variant(dtype, ins)
+        if ins is None:
+            st = Stage(stream)
+            ins = []
+            for b, beam in enumerate(beams):
+                ld = batch.min_ld[b]
+                d = abi.DeviceBuffer(beam["survey"].nchans * ld * isz)
+                st.fill(beam, d.ptr, ld, dtype)
+                ins.append((d, ld))
+            st.free()
+            abi.synchronize()
+        if v is None:
+            v = tune_variant(dtype, ins)
         batch.set_variant(v)
         # launch shape: one timed launch of each, keep the faster
         mode_ms = {}
@@ -310,6 +412,8 @@ def main():
             f"{B / (ms / 1e3) / 1e9:.1f} GB/s algorithmic")
         batch.mode_ms = {"persistent": mode_ms[0], "flat": mode_ms[1],
                          "chosen": "flat" if best == abi.BATCH_FLAT else "persistent"}
+        if args.precision:
+            capture_rows(dtype, ins)
         return batch, ins, ms, B, v
 
     batch16, ins16, ms16, B16, v16 = run_batch(abi.F16, "fp16")
@@ -380,12 +484,21 @@ def main():
         plain = {"ms": ms32, "dm_samples_per_s": total_work / ms32 * 1e3, "tile_variant": v32,
                  "hbm_frac": B32 / (ms32 / 1e3) / HBM_PEAK, "launches_per_step": len(plans)}
         log(f"[plain fp32] variant {v32}: {ms32:.1f} ms/step, {total_work / ms32 * 1e3:.3e} DM*samples/s")
+        # the same FP32 inputs batched into one launch (batching alone, no precision change)
+        batch32, _, ms32b, B32b, _ = run_batch(abi.F32, "fp32", ins=ins32, v=v32)
+        plain["batched_fp32"] = {"ms": ms32b, "dm_samples_per_s": total_work / ms32b * 1e3,
+                                 "launch_shape": batch32.mode_ms,
+                                 "hbm_frac": B32b / (ms32b / 1e3) / HBM_PEAK,
+                                 "vs_plain_fp32": ms32 / ms32b}
+        del batch32
         for d, _ in ins32:
             d.free()
 
     cpu, cpu_ref = (None, None) if args.no_cpu else cpu_baseline()
     if cpu:
         log(f"[cpu] {cpu}")
+
+    precision = precision_report(beams, prec_rows) if args.precision else None
 
     value = total_work / ms16 * 1e3
     achieved = B16 / (ms16 / 1e3)
@@ -409,7 +522,7 @@ def main():
         "storage": "f16",
         "data": "synthetic (seeded Philox noise, bandpass, 20 injected pulses per beam), generated on the device",
         "config": {"workload": f"{len(beams)} uneven beams, nchans 512/1024, nsamp 2^17..2^20, "
-                               "ndms 1024..4096, dmstep 0.1..0.5, one persistent launch",
+                               "ndms 1024..4096, dmstep 0.1..0.5, one launch",
                    "beams": len(beams), "sum_dm_samples": total_work, "sum_adds": total_adds,
                    "tile_variant": v16, "launch_shape": modes16},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
@@ -423,6 +536,7 @@ def main():
         "batched_fp16_vs_plain_fp32": (value / plain["dm_samples_per_s"]) if plain else None,
         "single_beam_fp32": result.get("single_fp32"),
         "spot_checks_bitexact": checks,
+        "precision": precision,
         "detection": det,
         "cpu_baseline": cpu,
         "cpu_reference_1thread": cpu_ref,
