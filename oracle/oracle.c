@@ -48,7 +48,10 @@
  *                      integer data, injected pulse recovered at its DM/width
  *   dbscan2d           scikit-learn DBSCAN(metric="chebyshev")
  *   dbscan_ref         hand-derived cases from the reference algorithm's
- *                      text (the reference needs Qt headers: unbuildable here)
+ *                      text, and scikit-learn DBSCAN where the neighbour
+ *                      graph is a union of cliques (the reference needs Qt
+ *                      headers: unbuildable here; chained neighbourhoods are
+ *                      pinned only by the hand cases)
  *   classify_ref       hand-derived cases (RFI and width rules, the
  *                      histogram smoothing and median, the model curve's
  *                      closed form at its peak); parity with the reference

@@ -128,6 +128,32 @@ def test_dbscan_ref_hand_cases():
     assert n == 0 and lab.tolist() == [-1, -1, -1]
 
 
+@pytest.mark.parametrize("min_points", [1, 2, 3, 4])
+def test_dbscan_ref_isolated_boxes_match_sklearn(min_points):
+    """Where the neighbour graph is a disjoint union of cliques (blobs that
+    fit in one range box, far apart) the reference's exclusion rules do not
+    matter and its DBSCAN reduces to the textbook one: a blob of more than
+    min_points points is one cluster, a smaller one is noise.  scikit-learn
+    on range-scaled coordinates (Chebyshev, eps 1, min_samples counting the
+    point itself = min_points + 1) then gives the same partition, with the
+    reference's cluster ids = scikit-learn's + 1 (both number clusters in
+    order of discovery).  Different ranges per dimension pin which range
+    scales which coordinate."""
+    from sklearn.cluster import DBSCAN
+    rng = np.random.default_rng(40 + min_points)
+    ranges = np.array([0.5, 2.0, 1.5])  # time, DM, S/N
+    pts = []
+    for k in range(40):
+        centre = np.array([k * 7.0, (k % 5) * 23.0, (k % 3) * 17.0]) * ranges
+        size = int(rng.integers(1, 7))
+        pts.append(centre + rng.uniform(0.0, 0.95, (size, 3)) * ranges)
+    pts = np.concatenate(pts)[rng.permutation(sum(len(p) for p in pts))]
+    lab, n = O.dbscan_ref(pts[:, 0], pts[:, 1], pts[:, 2], *ranges, min_points)
+    sk = DBSCAN(eps=1.0, min_samples=min_points + 1, metric="chebyshev").fit(pts / ranges).labels_
+    assert n == sk.max() + 1 > 0
+    assert np.array_equal(lab, np.where(sk < 0, -1, sk + 1))
+
+
 def test_median5_network_is_a_median():
     for perm in itertools.permutations([1.0, 2.0, 3.0, 4.0, 5.0]):
         assert O.median5(*perm) == 3.0
