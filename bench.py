@@ -10,7 +10,9 @@ launch with FP32 accumulation.  The same beams are also run with U8
 storage, through the plain path (FP32 storage, one launch per beam) for the
 batched/FP16 vs plain/FP32 ratio, and batched with FP32 storage; a single
 full beam (1024 channels, 2^20 samples, 4096 DMs at step 0.2) runs through
-the plain FP32 path.  For four beams, sampled DM rows of every storage type
+the plain FP32 path, and the per-chunk search pipeline (normalise, batched
+dedispersion, block statistics, detection) on 16 beams is timed eager and
+replayed from its hipGraph.  For four beams, sampled DM rows of every storage type
 are compared with the oracle (skip with --no-precision): bit-exact against
 the same-storage restatement, error against the FP64 sum of the
 unquantised input, and the candidate lists of those rows.
@@ -301,6 +303,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-beam FP32 pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the batched FP16 launches")
+    ap.add_argument("--no-pipeline", dest="pipeline", action="store_false",
+                    help="skip the graph-replayed search pipeline (normalise, dedisperse, detect) on 16 beams")
     ap.add_argument("--no-precision", dest="precision", action="store_false",
                     help="skip the comparison of sampled DM rows of four beams with the oracle "
                          "(FP64 error, bit-exact same-storage rows, candidate lists)")
@@ -311,7 +315,7 @@ def main():
         sys.exit(2)
     if args.profile:
         args.no_single = args.no_plain = args.no_cpu = True
-        args.precision = False
+        args.precision = args.pipeline = False
 
     stream_p = abi.C.c_void_p()
     abi.check(abi.lib().mdsm_stream_create(abi.C.byref(stream_p)))
@@ -494,6 +498,42 @@ def main():
         for d, _ in ins32:
             d.free()
 
+    pipeline = None
+    if args.pipeline:
+        # the per-chunk search sequence (normalise -> batched dedispersion ->
+        # block statistics -> detection) on the first 16 beams, eager and
+        # replayed from its hipGraph (the batch outputs are freed first)
+        for o in outs:
+            o.free()
+        nb = min(16, len(beams))
+        pipe = abi.Pipeline(surveys[:nb], abi.F16, stats_block=16384, nwidths=7, thresh=6.0,
+                            cand_cap=1 << 21)
+        raws = []
+        for b in range(nb):
+            sv = surveys[b]
+            _, S = abi.delay_table(sv)
+            ld = sv.nsamp + S
+            d = abi.DeviceBuffer(sv.nchans * ld * 4)
+            rng = np.random.default_rng(beams[b]["seed"])
+            gain = rng.uniform(0.5, 2.0, sv.nchans).astype(np.float32)
+            off = rng.uniform(0.0, 10.0, sv.nchans).astype(np.float32)
+            abi.generate(beams[b]["seed"], sv, 0, ld, d.ptr, ld, gain, off, beams[b]["pulses"], stream)
+            raws.append((d, ld))
+        abi.synchronize()
+        pipe.bind([d.ptr for d, _ in raws], [l for _, l in raws])
+        pipe.set_variant(v16)
+        eager = time_steps(lambda: pipe.run(stream, graph=False), args.steps, args.warmup, stream)
+        pipe.run(stream, graph=True)  # records the graph
+        graph = time_steps(lambda: pipe.run(stream, graph=True), args.steps, args.warmup, stream)
+        ncand = sum(len(pipe.candidates(b)) for b in range(nb))
+        work = sum(sv.ndms * sv.nsamp for sv in surveys[:nb])
+        pipeline = {"beams": nb, "storage": "f16", "eager_ms": eager, "graph_ms": graph,
+                    "dm_samples_per_s_graph": work / graph * 1e3, "candidates": ncand}
+        log(f"[pipeline fp16, {nb} beams] eager {eager:.1f} ms, graph {graph:.1f} ms, {ncand} candidates")
+        pipe.close()
+        for d, _ in raws:
+            d.free()
+
     cpu, cpu_ref = (None, None) if args.no_cpu else cpu_baseline()
     if cpu:
         log(f"[cpu] {cpu}")
@@ -537,6 +577,7 @@ def main():
         "single_beam_fp32": result.get("single_fp32"),
         "spot_checks_bitexact": checks,
         "precision": precision,
+        "pipeline_fp16": pipeline,
         "detection": det,
         "cpu_baseline": cpu,
         "cpu_reference_1thread": cpu_ref,
