@@ -66,7 +66,7 @@ def build_stream(cfg, seed, max_events, max_chunks=100000, log=print):
     return d_snip, events[:n], stats, time.time() - t0
 
 
-THRESHOLDS = (0.9, 0.5, 0.3, 0.2, 0.1, 0.05, 0.02, 0.01, 0.005, 0.002, 0.001)
+THRESHOLDS = (0.9, 0.5, 0.3, 0.2, 0.1, 0.05, 0.02, 0.01, 0.005, 0.002, 0.001, 0.0005, 0.0002, 0.0001)
 
 
 def recall_fp(prob, events, stats, t):
@@ -83,6 +83,24 @@ def operating_threshold(prob, events, stats, fp_fraction=0.25):
     budget = fp_fraction * stats["false_clusters"]
     ok = [t for t in THRESHOLDS if recall_fp(prob, events, stats, t)[1] <= budget]
     return min(ok) if ok else max(THRESHOLDS)
+
+
+def build_mixed(cfgs, seeds, max_events, log=print):
+    """One training stream drawn from several surveys: equal shares of
+    max_events from each (cfg, seed), appended into one snippet buffer."""
+    d_snip = abi.DeviceBuffer(max_events * ROW * 2)
+    events = np.zeros(max_events, abi.EVENT_DTYPE)
+    n, t0, stats = 0, time.time(), []
+    for i, (cfg, seed) in enumerate(zip(cfgs, seeds)):
+        quota = max_events * (i + 1) // len(cfgs)
+        sim = abi.TriageSim(cfg, seed)
+        while n < quota:
+            _, n = sim.run(50, d_snip.ptr, quota, events, n)
+            log(f"  stream seed {seed}: {sim.stats.chunks} chunks, {n} events, {time.time() - t0:.0f} s")
+        stats.append(sim.stats.as_dict())
+        sim.close()
+    total = {k: sum(st[k] for st in stats) for k in stats[0]}
+    return d_snip, events[:n], total, time.time() - t0
 
 
 def judge(prob, events, stats, thresholds):
@@ -105,13 +123,24 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--threshold", type=float, default=None,
                     help="operating threshold (default: picked on a validation stream)")
+    ap.add_argument("--single-survey", action="store_true",
+                    help="train on the single-beam survey only (default: three beam shapes)")
     ap.add_argument("--save", default=None, help="write trained parameters (.npy, canonical layout)")
     ap.add_argument("--json", default=None, help="write the results as JSON")
     args = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)  # noqa: E731
     res = {"config": vars(args)}
-    log(f"building the training stream ({args.events} events)")
-    d_tr, ev_tr, st_tr, t_sim = build_stream(make_config(TRAIN_SURVEY), args.seed, args.events, log=log)
+    if args.single_survey:
+        log(f"building the training stream ({args.events} events, one survey)")
+        d_tr, ev_tr, st_tr, t_sim = build_stream(make_config(TRAIN_SURVEY), args.seed, args.events, log=log)
+    else:
+        # equal shares from the single-beam survey and the two other beam
+        # shapes of the held-out streams (other seeds than the held-out ones)
+        shapes = [TRAIN_SURVEY, EVAL_SURVEYS[0], EVAL_SURVEYS[2]]
+        log(f"building the training stream ({args.events} events from {len(shapes)} beam shapes)")
+        d_tr, ev_tr, st_tr, t_sim = build_mixed([make_config(sv) for sv in shapes],
+                                                [args.seed + k for k in range(len(shapes))],
+                                                args.events, log=log)
     res["train"] = {"events": len(ev_tr), "pulses": int(ev_tr["label"].sum()), "stats": st_tr,
                     "sim_seconds": t_sim}
     lab = abi.DeviceBuffer.from_numpy(ev_tr["label"].astype(np.int32))
@@ -127,23 +156,35 @@ def main(argv=None):
         res["epochs"].append({"loss": loss, "seconds": dt, "events_per_s": len(ev_tr) / dt})
         log(f"epoch {ep}: loss {loss:.4f}, {len(ev_tr) / dt:.0f} events/s")
     del d_tr
-    # operating point from a validation stream of the training survey
-    d_va, ev_va, st_va, _ = build_stream(make_config(TRAIN_SURVEY), args.seed + 50, 400 * args.eval_chunks,
+    # operating point from a validation stream of the single-beam survey
+    d_va, ev_va, st_va, _ = build_stream(make_config(TRAIN_SURVEY), args.seed + 50, 2000 * args.eval_chunks,
                                          max_chunks=max(args.eval_chunks // 2, 1), log=log)
+    if len(ev_va) >= 2000 * args.eval_chunks:
+        raise RuntimeError("validation stream filled its event buffer; raise the capacity")
     d_prob = abi.DeviceBuffer(4 * max(len(ev_va), 1))
     abi.predict(net, d_va.ptr, len(ev_va), d_prob.ptr)
     abi.synchronize()
     pv = d_prob.download(np.float32, (len(ev_va),))
     thr = args.threshold if args.threshold is not None else operating_threshold(pv, ev_va, st_va)
+    # a second operating point at the north-star bound itself (half the
+    # plain path's false positives on the validation stream)
+    thr_half = operating_threshold(pv, ev_va, st_va, fp_fraction=0.5)
     res["validation"] = judge(pv, ev_va, st_va, THRESHOLDS)
     res["threshold"] = thr
-    log(f"operating threshold {thr}")
+    res["threshold_half"] = thr_half
+    log(f"operating thresholds {thr} (quarter of the plain false positives), {thr_half} (half)")
     del d_va, d_prob
-    thresholds = sorted({0.1, thr, 0.5, 0.9})
+    thresholds = THRESHOLDS
     res["eval"] = []
     for i, sv in enumerate(EVAL_SURVEYS):
-        d_te, ev_te, st_te, _ = build_stream(make_config(sv), args.seed + 100 + i, 400 * args.eval_chunks,
+        # room for every event of every chunk: a stream cut inside a chunk
+        # would count that chunk's pulses for the plain path but not for the
+        # network
+        cap = 2000 * args.eval_chunks
+        d_te, ev_te, st_te, _ = build_stream(make_config(sv), args.seed + 100 + i, cap,
                                              max_chunks=args.eval_chunks, log=log)
+        if len(ev_te) >= cap:
+            raise RuntimeError("held-out stream filled its event buffer; raise the capacity")
         n = len(ev_te)
         d_prob = abi.DeviceBuffer(4 * max(n, 1))
         t = time.time()
@@ -154,6 +195,7 @@ def main(argv=None):
         r = judge(prob, ev_te, st_te, thresholds)
         r["survey"] = sv
         r["operating_point"] = r[f"cnn@{thr}"]
+        r["operating_point_half"] = r[f"cnn@{thr_half}"]
         r["inference_events_per_s"] = n / max(t_inf, 1e-9)
         res["eval"].append(r)
         log(json.dumps(r))

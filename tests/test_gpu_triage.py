@@ -69,6 +69,9 @@ def _stream(seed, nchunks, max_events):
     ev, n = sim.run(nchunks, d_snip.ptr, max_events)
     stats = sim.stats.as_dict()
     sim.close()
+    # every cluster of every chunk is an event (a full buffer would drop some
+    # and bias the recall comparison)
+    assert n < max_events and stats["clusters"] == n
     return d_snip, ev[:n], stats
 
 
