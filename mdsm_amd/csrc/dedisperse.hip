@@ -535,7 +535,7 @@ __global__ __launch_bounds__(256) void dedisp_kernel(const typename Tile::T *__r
                                                      uint32_t ndms, uint32_t rw, uint32_t cbr,
                                                      uint32_t n_dtiles, uint32_t n_ttiles,
                                                      uint32_t xcd_remap, float *__restrict__ out) {
-  extern __shared__ float lds[];
+  extern __shared__ __attribute__((aligned(16))) float lds[];
   // DM tiles vary fastest: the workgroups resident at one time then share a
   // narrow time range of the input.  Workgroup IDs are dealt round-robin over
   // the 8 XCDs, each with its own L2; with xcd_remap, XCD x takes the time
@@ -607,15 +607,27 @@ __device__ __forceinline__ BeamItem decode_item(const BeamDesc *__restrict__ bea
   return it;
 }
 
+// (The dynamic LDS array is declared 16-byte aligned: it follows any static
+// LDS of the kernel, and the packed tiles access it with 8-byte operations.)
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | (uint64_t)uniform((uint32_t)v);
+}
+
+// Every tile parameter goes through readfirstlane: inside the persistent
+// kernel's item loop the compiler cannot prove the descriptor fields
+// wave-uniform and would otherwise keep the per-block address arithmetic in
+// VALU registers.
 template <typename Tile>
 __device__ __forceinline__ void run_item(const BeamDesc *__restrict__ beams, const BeamItem &it,
                                          uint32_t cbr, float *lds) {
   const BeamDesc &bd = *it.bd;
   char *base = reinterpret_cast<char *>(const_cast<BeamDesc *>(beams));
-  Tile::run(reinterpret_cast<const typename Tile::T *>(base + bd.in_off), bd.ld,
-            reinterpret_cast<const int32_t *>(base + bd.table_off),
-            reinterpret_cast<const int32_t *>(base + bd.rel_off), bd.nchans, bd.nsamp, bd.ndms,
-            bd.row_words, cbr, reinterpret_cast<float *>(base + bd.out_off), it.tt, it.dt, lds);
+  Tile::run(reinterpret_cast<const typename Tile::T *>(base + (int64_t)uniform64((uint64_t)bd.in_off)),
+            uniform64(bd.ld), reinterpret_cast<const int32_t *>(base + (int64_t)uniform64((uint64_t)bd.table_off)),
+            reinterpret_cast<const int32_t *>(base + (int64_t)uniform64((uint64_t)bd.rel_off)),
+            uniform(bd.nchans), uniform(bd.nsamp), uniform(bd.ndms), uniform(bd.row_words), cbr,
+            reinterpret_cast<float *>(base + (int64_t)uniform64((uint64_t)bd.out_off)), uniform(it.tt),
+            uniform(it.dt), lds);
 }
 
 // Persistent multi-beam kernel: resident workgroups pull items from eight
@@ -627,7 +639,7 @@ template <typename Tile>
 __global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__ beams,
                                                     uint32_t nbeams, uint32_t total_items,
                                                     uint32_t cbr, uint32_t *__restrict__ counter) {
-  extern __shared__ float lds[];
+  extern __shared__ __attribute__((aligned(16))) float lds[];
   __shared__ uint32_t item_s;
   const uint32_t slot = blockIdx.x & 7u;
   while (true) {
@@ -648,7 +660,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__
 template <typename Tile>
 __global__ __launch_bounds__(256) void batch_flat_kernel(const BeamDesc *__restrict__ beams,
                                                          uint32_t nbeams, uint32_t cbr) {
-  extern __shared__ float lds[];
+  extern __shared__ __attribute__((aligned(16))) float lds[];
   const BeamItem it = decode_item(beams, nbeams, blockIdx.x);
   if (!it.valid) return;  // whole workgroup, before any barrier
   run_item<Tile>(beams, it, cbr, lds);
