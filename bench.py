@@ -498,6 +498,20 @@ def main():
         for d, _ in ins32:
             d.free()
 
+    # measured device copy bandwidth (read + write bytes per second of a
+    # 4 GiB device-to-device copy): the attainable HBM rate beside the 8 TB/s
+    # nominal peak used for roofline.frac
+    copy_gbs = None
+    if not args.profile:
+        nb_copy = 4 << 30
+        src, dst = abi.DeviceBuffer(nb_copy), abi.DeviceBuffer(nb_copy)
+        ms_copy = time_steps(lambda: abi.check(abi.lib().mdsm_memcpy_dtod(dst.ptr, src.ptr, nb_copy, stream)),
+                             3, 1, stream)
+        copy_gbs = 2 * nb_copy / (ms_copy / 1e3) / 1e9
+        log(f"[peaks] device copy {copy_gbs:.0f} GB/s (read + write)")
+        src.free()
+        dst.free()
+
     pipeline = None
     if args.pipeline:
         # the per-chunk search sequence (normalise -> batched dedispersion ->
@@ -567,7 +581,7 @@ def main():
                    "tile_variant": v16, "launch_shape": modes16},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": B16},
+                     "algorithmic_bytes_per_launch": B16, "measured_copy_gbs": copy_gbs},
         "valu": {"adds_per_s": total_adds / (ms16 / 1e3), "peak_adds_per_s": VALU_ADD_PEAK,
                  "frac": total_adds / (ms16 / 1e3) / VALU_ADD_PEAK},
         "batched_u8": u8,
