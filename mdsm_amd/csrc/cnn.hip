@@ -43,10 +43,47 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ------------------------------------------------------------ loaders
+// Each loader fills 8 consecutive k of one operand row: fetchA(m, k, ...)
+// gives A(m, k..k+7), fetchB(n, k, ...) gives B(k..k+7, n); k is a multiple
+// of 8 and elements at or past kend are 0.  Index arithmetic is 32-bit
+// (every operand here has fewer than 2^31 elements) and a run of 8 elements
+// that is contiguous and 16-byte aligned in memory is one 16-byte load.
+__device__ __forceinline__ bool aligned16(const uint16_t *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+__device__ __forceinline__ void load8(const uint16_t *p, uint16_t v[8]) {
+  const uint4 q = *reinterpret_cast<const uint4 *>(p);
+  v[0] = (uint16_t)q.x; v[1] = (uint16_t)(q.x >> 16);
+  v[2] = (uint16_t)q.y; v[3] = (uint16_t)(q.y >> 16);
+  v[4] = (uint16_t)q.z; v[5] = (uint16_t)(q.z >> 16);
+  v[6] = (uint16_t)q.w; v[7] = (uint16_t)(q.w >> 16);
+}
+__device__ __forceinline__ void zero8(uint16_t v[8]) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = 0;
+}
+
 struct Dense {  // (r, c) at p[r*sr + c*sc]
   const uint16_t *p;
   int64_t sr, sc;
-  __device__ __forceinline__ uint16_t at(int64_t r, int64_t c) const { return p[r * sr + c * sc]; }
+  __device__ __forceinline__ void fetchA(int64_t m, int64_t k, int64_t kend, uint16_t v[8]) const {
+    const uint16_t *q = p + m * sr + k * sc;
+    if (sc == 1 && k + 8 <= kend && aligned16(q)) {
+      load8(q, v);
+      return;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = k + e < kend ? q[e * sc] : (uint16_t)0;
+  }
+  __device__ __forceinline__ void fetchB(int64_t n, int64_t k, int64_t kend, uint16_t v[8]) const {
+    const uint16_t *q = p + k * sr + n * sc;
+    if (sr == 1 && k + 8 <= kend && aligned16(q)) {
+      load8(q, v);
+      return;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = k + e < kend ? q[e * sr] : (uint16_t)0;
+  }
 };
 
 // im2col view of NHWC x[B][H][W][C]: (pixel, k) with k = (ky*3 + kx)*C + c
@@ -55,16 +92,47 @@ template <bool FLIP>
 struct Im2col {
   const uint16_t *p;
   int H, W, C;
-  __device__ __forceinline__ uint16_t at(int64_t pix, int64_t k) const {
-    const int kk = (int)(k / C), c = (int)(k - (int64_t)kk * C);
+  __device__ __forceinline__ uint16_t elem(int b, int h, int w, int k) const {
+    const int kk = k / C, c = k - kk * C;
     const int ky = kk / 3, kx = kk - 3 * ky;
-    const int w = (int)(pix % W);
-    const int64_t bh = pix / W;
-    const int h = (int)(bh % H);
-    const int64_t b = bh / H;
     const int hh = FLIP ? h - ky + 1 : h + ky - 1, ww = FLIP ? w - kx + 1 : w + kx - 1;
     if (hh < 0 || hh >= H || ww < 0 || ww >= W) return 0;
-    return p[((b * H + hh) * W + ww) * C + c];
+    return p[(int64_t)((b * H + hh) * W + ww) * C + c];
+  }
+  // one pixel, 8 consecutive k: with C a multiple of 8 they share (ky, kx)
+  // and are 8 consecutive channels, one aligned 16-byte load
+  __device__ __forceinline__ void fetchA(int64_t m, int64_t k, int64_t kend, uint16_t v[8]) const {
+    const int pix = (int)m, kk0 = (int)k;
+    const int w = pix % W, bh = pix / W, h = bh % H, b = bh / H;
+    if ((C & 7) == 0 && k + 8 <= kend) {
+      const int kk = kk0 / C, c = kk0 - kk * C;
+      const int ky = kk / 3, kx = kk - 3 * ky;
+      const int hh = FLIP ? h - ky + 1 : h + ky - 1, ww = FLIP ? w - kx + 1 : w + kx - 1;
+      if (hh < 0 || hh >= H || ww < 0 || ww >= W) {
+        zero8(v);
+      } else {
+        load8(p + (int64_t)((b * H + hh) * W + ww) * C + c, v);
+      }
+      return;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = k + e < kend ? elem(b, h, w, kk0 + e) : (uint16_t)0;
+  }
+  // 8 consecutive pixels, one k: stepped through (w, h, b) without division
+  __device__ __forceinline__ void fetchB(int64_t n, int64_t k, int64_t kend, uint16_t v[8]) const {
+    int pix = (int)k;
+    int w = pix % W, bh = pix / W, h = bh % H, b = bh / H;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[e] = k + e < kend ? elem(b, h, w, (int)n) : (uint16_t)0;
+      if (++w == W) {
+        w = 0;
+        if (++h == H) {
+          h = 0;
+          ++b;
+        }
+      }
+    }
   }
 };
 
@@ -72,9 +140,16 @@ struct Im2col {
 struct WFlip {
   const uint16_t *p;
   int Ci, Co;
-  __device__ __forceinline__ uint16_t at(int64_t k, int64_t n) const {
-    const int kk = (int)(k / Co), co = (int)(k - (int64_t)kk * Co);
-    return p[(int64_t)co * 9 * Ci + kk * Ci + n];
+  __device__ __forceinline__ void fetchB(int64_t n, int64_t k, int64_t kend, uint16_t v[8]) const {
+    int kk = (int)k / Co, co = (int)k - kk * Co;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[e] = k + e < kend ? p[(int64_t)co * 9 * Ci + kk * Ci + (int)n] : (uint16_t)0;
+      if (++co == Co) {
+        co = 0;
+        ++kk;
+      }
+    }
   }
 };
 
@@ -99,11 +174,8 @@ __global__ __launch_bounds__(NT) void gemm_kernel(LA A, LB B, int64_t M, int64_t
     {
       const int64_t m = m0 + lr;
       uint16_t v[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int64_t k = k0 + lk + e;
-        v[e] = (m < M && k < kend) ? A.at(m, k) : (uint16_t)0;
-      }
+      if (m < M) A.fetchA(m, k0 + lk, kend, v);
+      else zero8(v);
       uint16_t *dst = As + lr * PITCH + lk;
 #pragma unroll
       for (int e = 0; e < 8; ++e) dst[e] = v[e];
@@ -111,11 +183,8 @@ __global__ __launch_bounds__(NT) void gemm_kernel(LA A, LB B, int64_t M, int64_t
     {
       const int64_t n = n0 + lr;
       uint16_t v[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int64_t k = k0 + lk + e;
-        v[e] = (n < N && k < kend) ? B.at(k, n) : (uint16_t)0;
-      }
+      if (n < N) B.fetchB(n, k0 + lk, kend, v);
+      else zero8(v);
       uint16_t *dst = Bt + lr * PITCH + lk;
 #pragma unroll
       for (int e = 0; e < 8; ++e) dst[e] = v[e];
