@@ -315,13 +315,22 @@ __global__ void colsum_kernel(const uint16_t *__restrict__ x, int64_t rows, int 
   }
 }
 
-__global__ void sum_parts_kernel(const float *__restrict__ part, int nparts, int C,
-                                 float *__restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
+// one workgroup per channel: thread t sums parts t, t + NT, ... in order,
+// then a fixed binary tree (deterministic for a given nparts)
+__global__ __launch_bounds__(NT) void sum_parts_kernel(const float *__restrict__ part, int nparts,
+                                                       int C, float *__restrict__ out) {
+  __shared__ float s[NT];
+  const int c = blockIdx.x;
   float v = 0.f;
-  for (int p = 0; p < nparts; ++p) v += part[(int64_t)p * C + c];
-  out[c] = v;
+  for (int p = threadIdx.x; p < nparts; p += NT) v += part[(int64_t)p * C + c];
+  s[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int h = NT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = s[0];
 }
 
 // mean softmax cross-entropy over B rows of 2 logits; dl = (p - onehot)/B (BF16)
@@ -469,8 +478,8 @@ int column_sums(const uint16_t *x, int64_t rows, int C, float *part, int max_par
   hipLaunchKernelGGL(colsum_kernel, dim3(nparts > 0 ? nparts : 1), dim3(NT), 0, s, x, rows, C, per,
                      part);
   MDSM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(sum_parts_kernel, dim3((C + NT - 1) / NT), dim3(NT), 0, s, part,
-                     nparts > 0 ? nparts : 1, C, out);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(C), dim3(NT), 0, s, part, nparts > 0 ? nparts : 1, C,
+                     out);
   MDSM_HIP(hipGetLastError());
   return MDSM_OK;
 }
