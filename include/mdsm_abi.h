@@ -137,8 +137,10 @@ int mdsm_plan_autotune(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_o
  * d_in dtype is the plan's: F32 floats, F16 halves, or U8 codes whose value
  * is u*(8/255) - 4 in normalised units (then the sum is exact in integers and
  * out = (float)(S*(8/255) - 4*nchans)).  Rows must start on 4-byte
- * boundaries: ld a multiple of 2 (F16) or 4 (U8); mdsm_plan_info reports the
- * smallest valid ld. */
+ * boundaries: ld a multiple of 2 (F16) or 4 (U8), rows shorter than 2 GiB;
+ * mdsm_plan_info reports the smallest valid ld.  A plan (like the
+ * reference's per-beam thread state) is used by one host thread at a time:
+ * the first call with a tile variant builds that variant's offset table. */
 int mdsm_dedisperse(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out, void *stream);
 
 /* --------------------------------------------------- bandpass normalisation
