@@ -689,7 +689,9 @@ template <> struct VariantDef<11> { using F32 = F32Tile<4, 16, 4>;  using F16 = 
 // 8 or 6 packed adds
 template <> struct VariantDef<12> { using F32 = F32Tile<12, 6, 4>;  using F16 = F16Tile<12, 4, 4>;  using U8 = U8Tile<12, 4, 4>; };
 template <> struct VariantDef<13> { using F32 = F32Tile<16, 6, 4>;  using F16 = F16Tile<16, 3, 4>;  using U8 = U8Tile<16, 3, 4>; };
-constexpr int kNumVariants = 13;
+// 16 trials per wave over 8-channel blocks: half the barriers per add
+template <> struct VariantDef<14> { using F32 = F32Tile<16, 4, 8>;  using F16 = F16Tile<16, 2, 8>;  using U8 = U8Tile<16, 2, 8>; };
+constexpr int kNumVariants = 14;
 
 template <typename V, typename Fn>
 int pick_dtype(int dtype, Fn &&fn) {
@@ -718,6 +720,7 @@ int with_tile(int dtype, int variant, Fn &&fn) {
     case 11: return pick_dtype<VariantDef<11>>(dtype, fn);
     case 12: return pick_dtype<VariantDef<12>>(dtype, fn);
     case 13: return pick_dtype<VariantDef<13>>(dtype, fn);
+    case 14: return pick_dtype<VariantDef<14>>(dtype, fn);
   }
   set_error("unknown dedispersion variant %d", variant);
   return MDSM_ERR_ARG;
