@@ -40,7 +40,7 @@ from mdsm_amd import abi  # noqa: E402
 
 HBM_PEAK = 8.0e12        # MI355X HBM3E, bytes/s
 VALU_ADD_PEAK = 39.3e12  # FP32 vector adds/s, one per lane per clock (157.3 TFLOP/s counts
-                         # packed FMA as 4 flops; packed FP32 is not used by this build)
+                         # packed FMA as 4 flops; packed FP32 is not used here)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch_fp16.json")
 FCH1, BAND, TSAMP = 1500.0, 300.0, 64e-6
 
@@ -310,8 +310,8 @@ def main():
                          "(FP64 error, bit-exact same-storage rows, candidate lists)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the path would shard by beam; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: this implementation has no multi-GPU path "
+              "(the path would shard by beam, one process per GPU)", file=sys.stderr)
         sys.exit(2)
     if args.profile:
         args.no_single = args.no_plain = args.no_cpu = True

@@ -42,12 +42,12 @@ enum {
 enum { MDSM_F32 = 0, MDSM_F16 = 1, MDSM_U8 = 2 };
 
 /* Delay-table conventions.
- * NORTHSTAR: D[c][d] = round(4148.808*DM_d*(f_c^-2 - f_top^-2)/tsamp), FP64.
+ * FP64_ROUND: D[c][d] = round(4148.808*DM_d*(f_c^-2 - f_top^-2)/tsamp), FP64.
  * REFERENCE: (int)(dm_shifts[c]*(lowdm + d*dmstep)/tsamp) in float with
  *   dm_shifts[c] = (float)(4148.741601*(F1^-2 - F2^-2)), F1 = fch1 + foff*c in
  *   float (multibeam_dedispersion_manager.cpp:325-326,366-369;
  *   multibeam_dedispersion_thread.cu:551-553). */
-enum { MDSM_DELAY_NORTHSTAR = 0, MDSM_DELAY_REFERENCE = 1 };
+enum { MDSM_DELAY_FP64_ROUND = 0, MDSM_DELAY_REFERENCE = 1 };
 
 /* Survey parameters of one beam: the <observation> XML fields of the
  * reference (SURVEY/BEAM, src/mdsm-multibeam/survey.h:8-88). */

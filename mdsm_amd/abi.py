@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libmdsm_amd.so")
 (MDSM_OK, MDSM_ERR_ARG, MDSM_ERR_HIP, MDSM_ERR_NOMEM, MDSM_ERR_OVERFLOW, MDSM_ERR_STATE,
  MDSM_ERR_IO, MDSM_ERR_FORMAT) = range(8)
 F32, F16, U8 = 0, 1, 2
-DELAY_NORTHSTAR, DELAY_REFERENCE = 0, 1
+DELAY_FP64_ROUND, DELAY_REFERENCE = 0, 1
 _ITEMSIZE = {F32: 4, F16: 2, U8: 1}
 _NP = {F32: np.float32, F16: np.uint16, U8: np.uint8}
 ALIGN = {F32: 1, F16: 2, U8: 4}  # required multiple of the input row length
@@ -36,7 +36,7 @@ class Survey(C.Structure):
                 ("ndms", C.c_uint32), ("nsamp", C.c_uint32), ("delay_mode", C.c_int32)]
 
     @classmethod
-    def make(cls, fch1, foff, nchans, tsamp, lowdm, dmstep, ndms, nsamp, delay_mode=DELAY_NORTHSTAR):
+    def make(cls, fch1, foff, nchans, tsamp, lowdm, dmstep, ndms, nsamp, delay_mode=DELAY_FP64_ROUND):
         return cls(fch1, foff, tsamp, lowdm, dmstep, nchans, ndms, nsamp, delay_mode)
 
     def as_dict(self):
@@ -438,7 +438,7 @@ class Pulse:
 
 
 def pulse_delays(s: Survey, dms):
-    """Per-channel delays (north-star formula) for arbitrary DMs, via the
+    """Per-channel delays (FP64 rounded formula) for arbitrary DMs, via the
     library's own table builder (a one-trial survey per DM)."""
     out = np.zeros((len(dms), s.nchans), np.int64)
     for i, dm in enumerate(dms):

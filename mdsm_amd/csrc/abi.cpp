@@ -29,7 +29,7 @@ void set_error(const char *fmt, ...) {
   va_end(ap);
 }
 
-// North-star delay: round(4148.808*DM*(f^-2 - ftop^-2)/tsamp) in FP64, the
+// FP64 rounded delay: round(4148.808*DM*(f^-2 - ftop^-2)/tsamp) in FP64, the
 // expression evaluated in exactly this order (built with -ffp-contract=off).
 static int32_t delay_ns(double fch1, double foff, uint32_t c, double tsamp, double dm) {
   const double f = fch1 + foff * (double)c;
@@ -49,7 +49,7 @@ int build_delay_table(const mdsm_survey_t *s, std::vector<int32_t> &table, uint3
   MDSM_REQUIRE(s->lowdm >= 0.0 && s->dmstep >= 0.0, MDSM_ERR_ARG, "DMs must be non-negative");
   const uint64_t n = (uint64_t)s->nchans * s->ndms;
   table.resize(n);
-  if (s->delay_mode == MDSM_DELAY_NORTHSTAR) {
+  if (s->delay_mode == MDSM_DELAY_FP64_ROUND) {
     for (uint32_t c = 0; c < s->nchans; ++c)
       for (uint32_t d = 0; d < s->ndms; ++d)
         table[(uint64_t)c * s->ndms + d] =

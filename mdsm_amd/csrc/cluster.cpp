@@ -1,6 +1,6 @@
 // cluster.cpp -- host-side candidate clustering.
 //
-// mdsm_dbscan2d: the north-star 2-D DBSCAN on (DM index, sample) with a
+// mdsm_dbscan2d: the 2-D DBSCAN on (DM index, sample) with a
 // Chebyshev neighbourhood, O(n (2 eps + 1) log n) through a row index over
 // candidates sorted by (dm_idx, sample).
 // mdsm_dbscan_ref: the reference's 3-D box DBSCAN

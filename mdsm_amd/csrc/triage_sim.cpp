@@ -5,7 +5,7 @@
 // Each chunk: host draws pulses (DM, width, S/N, arrival), zero-DM spikes
 // and narrowband RFI items; the device generates the raw rows (Philox
 // noise, bandpass gain/offset, dispersed pulses), adds the RFI, normalises
-// per channel, dedisperses (FP32 plan, north-star delays), computes
+// per channel, dedisperses (FP32 plan, FP64 rounded delays), computes
 // per-block statistics and boxcar candidates; the host sorts, clusters
 // (2-D DBSCAN), takes cluster peaks, labels each peak by matching it to the
 // injected pulses, and the device cuts the two snippets of every event.
@@ -88,7 +88,7 @@ extern "C" int mdsm_triage_sim_create(const mdsm_triage_config_t *cfg, uint64_t 
   s->cfg = *cfg;
   s->seed = seed;
   mdsm_survey_t sv = cfg->survey;
-  sv.delay_mode = MDSM_DELAY_NORTHSTAR;
+  sv.delay_mode = MDSM_DELAY_FP64_ROUND;
   s->cfg.survey = sv;
   int rc = mdsm_plan_create(&sv, MDSM_F32, &s->plan);
   if (rc != MDSM_OK) {

@@ -9,7 +9,7 @@ on/off channels).  Every cluster peak becomes an event with two BF16
 snippets (64 DM trials x 256 samples; 64 subbands x 256 samples
 dedispersed at the event DM) and a label (1 when it matches an injected
 pulse).  The network trains on the training stream with the library's own
-MFMA forward/backward kernels and is judged on held-out streams against
+MFMA forward/backward kernels and is evaluated on held-out streams against
 the plain path: recall on injected pulses and false-positive clusters.
 
     python -m mdsm_amd.triage_train --events 200000 --epochs 6 --json out.json
@@ -103,7 +103,7 @@ def build_mixed(cfgs, seeds, max_events, log=print):
     return d_snip, events[:n], total, time.time() - t0
 
 
-def judge(prob, events, stats, thresholds):
+def score(prob, events, stats, thresholds):
     """Plain vs learned triage on one held-out stream."""
     out = {"plain": {"recall": stats["injected_found"] / stats["injected"], "false_positives":
                      stats["false_clusters"], "clusters": stats["clusters"]}}
@@ -166,10 +166,10 @@ def main(argv=None):
     abi.synchronize()
     pv = d_prob.download(np.float32, (len(ev_va),))
     thr = args.threshold if args.threshold is not None else operating_threshold(pv, ev_va, st_va)
-    # a second operating point at the north-star bound itself (half the
+    # a second operating point at the target bound itself (half the
     # plain path's false positives on the validation stream)
     thr_half = operating_threshold(pv, ev_va, st_va, fp_fraction=0.5)
-    res["validation"] = judge(pv, ev_va, st_va, THRESHOLDS)
+    res["validation"] = score(pv, ev_va, st_va, THRESHOLDS)
     res["threshold"] = thr
     res["threshold_half"] = thr_half
     log(f"operating thresholds {thr} (quarter of the plain false positives), {thr_half} (half)")
@@ -192,7 +192,7 @@ def main(argv=None):
         abi.synchronize()
         t_inf = time.time() - t
         prob = d_prob.download(np.float32, (n,))
-        r = judge(prob, ev_te, st_te, thresholds)
+        r = score(prob, ev_te, st_te, thresholds)
         r["survey"] = sv
         r["operating_point"] = r[f"cnn@{thr}"]
         r["operating_point_half"] = r[f"cnn@{thr_half}"]

@@ -1,6 +1,6 @@
 """CPU oracle for the dedispersion + single-pulse detection path.
 
-TEST INFRASTRUCTURE ONLY: this package is the checker.  It may be imported by
+TEST INFRASTRUCTURE ONLY: this package is the oracle.  It may be imported by
 ``tests/``, by ``__graft_entry__.smoke()`` and by the CPU-baseline leg of
 ``bench.py``; the product package ``mdsm_amd`` never imports it.
 

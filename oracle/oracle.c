@@ -2,7 +2,7 @@
  * oracle.c -- CPU restatement of MDSM's brute-force dedispersion and
  * single-pulse detection path.  TEST INFRASTRUCTURE ONLY.
  *
- * This file is the checker.  Only the tests, the smoke check and the CPU
+ * This file is the oracle the tests compare with.  Only the tests, the smoke check and the CPU
  * baseline leg of the benchmark may load it; the product library never links
  * or calls it.  Everything here is plain C (gcc, OpenMP), written for clarity
  * and for bit-exact reproducibility, not for speed (except
@@ -30,7 +30,7 @@
  *   corner turn into beams      src/MDSM.cpp:71-104
  *   bandpass / RFI stage        src/mdsm-multibeam/multibeam_dedispersion_kernel.cu:302-450,
  *                               src/mdsm-multibeam/multibeam_dedispersion_thread.cu:224-390
- * North-star extensions restated from their own definitions (no reference
+ * Extensions beyond the reference restated from their own definitions (no reference
  * code exists for them): FP64 round() delay table with 4148.808, per-block
  * FP64 mean/RMS, dyadic boxcar detection, 2-D Chebyshev DBSCAN, FP16/uint8
  * input storage, Philox4x32-10 synthetic data.
@@ -81,7 +81,7 @@
 #define FLT_MAX_F 3.40282346638528859812e+38F
 #define FLT_MIN_F 1.17549435082228750797e-38F
 #define M_PI_D 3.14159265358979323846
-#define KDM_NORTHSTAR 4148.808
+#define KDM_FP64_ROUND 4148.808
 #define KDM_REFERENCE 4148.741601
 #define R256 256
 
@@ -141,11 +141,11 @@ void oracle_channel_bandpass(uint64_t seed, int nchans, double g_lo, double g_hi
   }
 }
 
-/* Delay of channel c for an arbitrary DM (north-star formula, FP64, round). */
+/* Delay of channel c for an arbitrary DM (4148.808, FP64, round). */
 static inline int64_t ns_delay(double fch1, double foff, int c, double tsamp, double dm) {
   double f = fch1 + foff * (double)c;
   double k = 1.0 / (f * f) - 1.0 / (fch1 * fch1);
-  return (int64_t)round(((KDM_NORTHSTAR * dm) * k) / tsamp);
+  return (int64_t)round(((KDM_FP64_ROUND * dm) * k) / tsamp);
 }
 
 /*
@@ -180,7 +180,7 @@ void oracle_generate(uint64_t seed, int nchans, double fch1, double foff, double
 /* ------------------------------------------------------------------ */
 /* Delay tables                                                        */
 /* ------------------------------------------------------------------ */
-/* North-star table: D[c][d] = round(4148.808*DM_d*(f_c^-2 - f_top^-2)/tsamp), FP64,
+/* FP64 rounded table: D[c][d] = round(4148.808*DM_d*(f_c^-2 - f_top^-2)/tsamp), FP64,
  * f_c = fch1 + foff*c (channel 0 = top), DM_d = lowdm + d*dmstep. */
 void oracle_delay_table(double fch1, double foff, int nchans, double tsamp, double lowdm,
                         double dmstep, int ndms, int32_t *table) {
@@ -189,7 +189,7 @@ void oracle_delay_table(double fch1, double foff, int nchans, double tsamp, doub
     double k = 1.0 / (f * f) - 1.0 / (fch1 * fch1);
     for (int d = 0; d < ndms; ++d) {
       double dm = lowdm + dmstep * (double)d;
-      table[(int64_t)c * ndms + d] = (int32_t)round(((KDM_NORTHSTAR * dm) * k) / tsamp);
+      table[(int64_t)c * ndms + d] = (int32_t)round(((KDM_FP64_ROUND * dm) * k) / tsamp);
     }
   }
 }
@@ -500,7 +500,7 @@ void oracle_normalise_u8(const float *raw, int64_t ld, int nchans, int64_t n, co
 }
 
 /* ------------------------------------------------------------------ */
-/* Detection (north star): per-block FP64 mean/RMS, dyadic boxcars,      */
+/* Detection (own definition): per-block FP64 mean/RMS, dyadic boxcars, */
 /* best width per sample, threshold.                                     */
 /* ------------------------------------------------------------------ */
 /* Block (d, b) covers samples [b*L, min((b+1)*L, nsamp)) of row d. */
@@ -597,7 +597,7 @@ int64_t oracle_threshold_ref(const float *x, int ndms, int nsamp, double mean, d
 }
 
 /* ------------------------------------------------------------------ */
-/* DBSCAN, 2-D north-star variant: points (d, t), Chebyshev distance,    */
+/* DBSCAN, 2-D variant: points (d, t), Chebyshev distance,              */
 /* neighbour iff |dd| <= eps and |dt| <= eps, core iff the neighbourhood  */
 /* (self included) has >= minpts points.  Clusters numbered 0.. in order  */
 /* of their first core point; a border point joins the lowest-numbered    */

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-@pytest.mark.parametrize("mode", [abi.DELAY_NORTHSTAR, abi.DELAY_REFERENCE])
+@pytest.mark.parametrize("mode", [abi.DELAY_FP64_ROUND, abi.DELAY_REFERENCE])
 @pytest.mark.parametrize("cfg", [
     (418.0, -0.078125, 256, 51.2e-6, 0.0, 0.1, 256),
     (1500.0, -0.29296875, 1024, 64e-6, 0.0, 0.2, 4096),
@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 def test_delay_table_matches_oracle(cfg, mode):
     fch1, foff, C, ts, lo, st, M = cfg
     t, ms = abi.delay_table(abi.Survey.make(fch1, foff, C, ts, lo, st, M, 1024, mode))
-    ref = (O.delay_table if mode == abi.DELAY_NORTHSTAR else O.delay_table_ref)(*cfg)
+    ref = (O.delay_table if mode == abi.DELAY_FP64_ROUND else O.delay_table_ref)(*cfg)
     assert np.array_equal(t, ref)
     assert ms == ref.max()
 

@@ -22,7 +22,7 @@ def _say(*a):
     print(*a, flush=True, file=sys.stderr)
 
 
-def _survey(nsamp, mode=abi.DELAY_NORTHSTAR, **kw):
+def _survey(nsamp, mode=abi.DELAY_FP64_ROUND, **kw):
     p = dict(CFG0)
     p.update(kw)
     return abi.Survey.make(p["fch1"], p["foff"], p["nchans"], p["tsamp"], p["lowdm"], p["dmstep"],

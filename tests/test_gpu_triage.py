@@ -102,7 +102,7 @@ def _predict(net, d, n):
 def test_learned_triage_beats_plain_false_positives():
     d_tr, ev_tr, st_tr = _stream(1, 40, 20000)
     d_va, ev_va, st_va = _stream(3, 10, 20000)  # validation: picks the operating threshold
-    d_te, ev_te, st_te = _stream(2, 15, 20000)  # held out: judged
+    d_te, ev_te, st_te = _stream(2, 15, 20000)  # held out: evaluated
     lab = abi.DeviceBuffer.from_numpy(ev_tr["label"].astype(np.int32))
     net = abi.TriageNet(64, 256, 256)
     net.init(3)
