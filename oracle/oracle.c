@@ -2,8 +2,9 @@
  * oracle.c -- CPU restatement of MDSM's brute-force dedispersion and
  * single-pulse detection path.  TEST INFRASTRUCTURE ONLY.
  *
- * This file is the oracle the tests compare with.  Only the tests, the smoke check and the CPU
- * baseline leg of the benchmark may load it; the product library never links
+ * This file is the oracle the tests compare with.  Only the tests, the
+ * smoke check and the CPU baseline leg of the benchmark may load it; the
+ * product library never links
  * or calls it.  Everything here is plain C (gcc, OpenMP), written for clarity
  * and for bit-exact reproducibility, not for speed (except
  * oracle_dedisperse_f32seq, which doubles as the multithreaded CPU baseline).
