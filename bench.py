@@ -293,6 +293,65 @@ def precision_report(beams, prec_rows, L=16384, thr=6.0, nwidths=7):
     return out
 
 
+COMPAT_XML = """<observation>
+  <dm lowDM="0" numDMs="{ndms}" dmStep="0.2" />
+  <channels nchans="1024" npols="1" />
+  <timing tsamp="0.000064" />
+  <samples nsamp="{nsamp}" nbits="32" />
+  <detection detectionThreshold="6" />
+  <writer filePrefix="compat" baseDirectory="{dir}" singleFileMode="1" />
+  <beams><beam beamId="0" topFrequency="1500" frequencyOffset="-0.29296875" /></beams>
+</observation>"""
+
+
+def compat_throughput(nsamp=1 << 18, ndms=4096, nchunks=6):
+    """The drop-in manager API (get_buffer_pointer / next_chunk_multibeam /
+    start_processing) driven as src/MDSM.cpp drives it, host buffers in and
+    out: per chunk the input upload (PCIe), the overlap-save copy,
+    dedispersion, the output download (PCIe) and the threshold detection.
+    Returns DM*samples/s over the steady-state chunks (the caller's own
+    filling of the input buffer excluded)."""
+    import ctypes as C
+    import tempfile
+    lib = abi.lib()
+    lib.mdsm_compat_open.argtypes = [C.c_char_p, C.c_uint]
+    lib.mdsm_compat_survey.argtypes = [C.POINTER(C.c_uint)] * 5
+    lib.mdsm_compat_buffer.restype = C.POINTER(C.c_float)
+    lib.mdsm_compat_next_chunk.argtypes = [C.c_uint, C.POINTER(C.c_uint), C.c_double, C.c_double]
+    lib.mdsm_compat_next_chunk.restype = C.POINTER(C.POINTER(C.c_float))
+    lib.mdsm_compat_start.argtypes = [C.c_uint]
+    lib.mdsm_compat_last_error.restype = C.c_char_p
+    with tempfile.TemporaryDirectory() as d:
+        xml = os.path.join(d, "obs.xml")
+        with open(xml, "w") as f:
+            f.write(COMPAT_XML.format(ndms=ndms, nsamp=nsamp, dir=d))
+        if lib.mdsm_compat_open(xml.encode(), 0) != 0:
+            raise RuntimeError(lib.mdsm_compat_last_error().decode())
+        nb, nch, ns, nd, ms = (C.c_uint() for _ in range(5))
+        lib.mdsm_compat_survey(C.byref(nb), C.byref(nch), C.byref(ns), C.byref(nd), C.byref(ms))
+        Cn, N, M, S = nch.value, ns.value, nd.value, ms.value
+        x = np.random.default_rng(5).standard_normal((Cn, N)).astype(np.float32)
+        times = []
+        for i in range(nchunks + 3):
+            buf = lib.mdsm_compat_buffer()
+            n = S if i == 0 else (N if i <= nchunks else 0)
+            if n:
+                np.ctypeslib.as_array(buf, shape=(Cn, n))[:] = x[:, :n]
+            samples = C.c_uint()
+            t0 = time.perf_counter()
+            lib.mdsm_compat_next_chunk(n, C.byref(samples), 64e-6 * i * N, 64e-6)
+            r = lib.mdsm_compat_start(n)
+            times.append(time.perf_counter() - t0)
+            if r == 0:
+                break
+        lib.mdsm_compat_close()
+    steady = times[3:nchunks + 1]
+    per = float(np.median(steady))
+    return {"workload": f"1 beam, 1024 ch x {M} DMs (step 0.2) x {N} samples per chunk, host buffers",
+            "s_per_chunk": per, "dm_samples_per_s": M * N / per,
+            "pcie_bytes_per_chunk": Cn * N * 4 + M * N * 4, "chunks_timed": len(steady)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +362,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-beam FP32 pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the batched FP16 launches")
+    ap.add_argument("--compat", action="store_true",
+                    help="only measure the drop-in manager API with host buffers (PCIe included)")
     ap.add_argument("--no-pipeline", dest="pipeline", action="store_false",
                     help="skip the graph-replayed search pipeline (normalise, dedisperse, detect) on 16 beams")
     ap.add_argument("--no-precision", dest="precision", action="store_false",
@@ -313,6 +374,11 @@ def main():
         print("bench.py runs on one GPU only: this implementation has no multi-GPU path "
               "(the path would shard by beam, one process per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.compat:
+        res = compat_throughput()
+        log(f"[compat] {res}")
+        print(json.dumps({"compat_host_buffers": res}))
+        return
     if args.profile:
         args.no_single = args.no_plain = args.no_cpu = True
         args.precision = args.pipeline = False
