@@ -352,6 +352,31 @@ def compat_throughput(nsamp=1 << 18, ndms=4096, nchunks=6):
             "pcie_bytes_per_chunk": Cn * N * 4 + M * N * 4, "chunks_timed": len(steady)}
 
 
+def tile_run(spec, steps):
+    """One dedispersion tile shape on the single-beam survey truncated to
+    2^18 samples (1024 channels x 4096 trials of 0.2), `steps` launches: the
+    workload the per-tile rocprofv3 counter passes profile.  spec is
+    '<f32|f16|u8>:<variant>'."""
+    name, v = spec.split(":")
+    dt = {"f32": abi.F32, "f16": abi.F16, "u8": abi.U8}[name]
+    s = abi.Survey.make(FCH1, -BAND / 1024, 1024, TSAMP, 0.0, 0.2, 4096, 1 << 18)
+    plan = abi.Plan(s, dt)
+    isz = {abi.F32: 4, abi.F16: 2, abi.U8: 1}[dt]
+    din = abi.DeviceBuffer(1024 * plan.min_ld * isz)
+    din.zero()
+    dout = abi.DeviceBuffer(4096 * (1 << 18) * 4)
+    plan.set_variant(int(v))
+    plan.dedisperse(din.ptr, plan.min_ld, dout.ptr)
+    abi.synchronize()
+    t = abi.Timer()
+    t.start()
+    for _ in range(steps):
+        plan.dedisperse(din.ptr, plan.min_ld, dout.ptr)
+    ms = t.stop() / steps
+    adds = 1024 * 4096 * (1 << 18)
+    return {"tile": spec, "ms": ms, "adds_per_s": adds / (ms / 1e3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,6 +387,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-beam FP32 pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the batched FP16 launches")
+    ap.add_argument("--tile", default=None,
+                    help="only time one tile shape ('<f32|f16|u8>:<variant>') at 1024 ch x 2^18 x 4096, "
+                         "the workload of the per-tile counter profiles")
     ap.add_argument("--compat", action="store_true",
                     help="only measure the drop-in manager API with host buffers (PCIe included)")
     ap.add_argument("--no-pipeline", dest="pipeline", action="store_false",
@@ -374,6 +402,11 @@ def main():
         print("bench.py runs on one GPU only: this implementation has no multi-GPU path "
               "(the path would shard by beam, one process per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.tile:
+        res = tile_run(args.tile, args.steps)
+        log(f"[tile] {res}")
+        print(json.dumps({"tile": res}))
+        return
     if args.compat:
         res = compat_throughput()
         log(f"[compat] {res}")
