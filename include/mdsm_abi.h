@@ -178,6 +178,27 @@ int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, float t
                        mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count, double *h_mean,
                        double *h_rms, void *stream);
 
+/* Legacy single-beam detector (process_brute,
+ * src/mdsm/dedispersion_output.cpp:135-213, one thread's share).  For each
+ * row k (DM lowdm + k*dmstep, FP32) and sample l < nsamp - 4 the median of
+ * samples l..l+4 (the selection network of dedispersion_output.cpp:14-53) is
+ * a candidate iff DM > 1 and
+ *   d_mean == NULL: median - mean >= rms*thresh, snr = median / rms, where
+ *     mean and rms come from FP64 sums over 32768-value blocks in the
+ *     reference's order and types (its last block is left out when there is
+ *     more than one); h_mean/h_rms receive them;
+ *   d_mean != NULL: median - d_mean[l/spc] >= d_rms[l/spc]*thresh, snr =
+ *     (median - d_mean[l/spc]) / d_rms[l/spc], with nstats entries in each
+ *     device array (at least (nsamp-5)/spc + 1); h_mean/h_rms untouched.
+ * All comparisons in FP32 as the reference rounds them.  ndms*nsamp < 2^32.
+ * *d_count receives the number of candidates (may exceed cap; order not
+ * defined). */
+int mdsm_detect_legacy(const float *d_x, uint32_t ndms, uint32_t nsamp, float lowdm,
+                       float dmstep, float thresh, const float *d_mean, const float *d_rms,
+                       uint32_t samples_per_chunk, uint32_t nstats, mdsm_cand_t *d_cands,
+                       uint32_t cap, uint32_t *d_count, float *h_mean, float *h_rms,
+                       void *stream);
+
 /* 5-tap running median per row, edges replicated (median_filter,
  * multibeam_dedispersion_kernel.cu:98-157), out of place. */
 int mdsm_median_filter5(const float *d_x, uint32_t nrows, uint32_t nsamp, float *d_out,

@@ -499,6 +499,65 @@ extern "C" int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsa
   return launch_threshold_global(d_x, nrows, nsamp, mean, rms, thresh, d_cands, cap, d_count, s);
 }
 
+extern "C" int mdsm_detect_legacy(const float *d_x, uint32_t ndms, uint32_t nsamp, float lowdm,
+                                  float dmstep, float thresh, const float *d_mean,
+                                  const float *d_rms, uint32_t samples_per_chunk, uint32_t nstats,
+                                  mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count,
+                                  float *h_mean, float *h_rms, void *stream) {
+  MDSM_REQUIRE(d_x && d_cands && d_count && ndms > 0 && nsamp > 0 && ndms <= 65535, MDSM_ERR_ARG,
+               "bad arguments");
+  // the reference counts values in an unsigned int
+  MDSM_REQUIRE((uint64_t)ndms * nsamp < (1ull << 32), MDSM_ERR_ARG,
+               "ndms * nsamp must stay below 2^32");
+  MDSM_REQUIRE(!d_mean == !d_rms, MDSM_ERR_ARG, "give both per-chunk mean and rms, or neither");
+  if (d_mean) {
+    MDSM_REQUIRE(samples_per_chunk > 0, MDSM_ERR_ARG, "samples_per_chunk must be positive");
+    const uint32_t last = nsamp > 4 ? nsamp - 5 : 0;
+    MDSM_REQUIRE((uint64_t)nstats >= (uint64_t)(last / samples_per_chunk) + 1, MDSM_ERR_ARG,
+                 "too few per-chunk statistics for nsamp");
+  }
+  hipStream_t s = as_stream(stream);
+  float gmean = 0.0f, grms = 0.0f;
+  if (!d_mean) {
+    // dedispersion_output.cpp:145-166: blocks of mod_factor values; the loop
+    // leaves after block n once (n + 1) * mod_factor >= vals, so with more
+    // than one block the last block (full or partial) never enters the mean
+    const uint32_t vals = ndms * nsamp;
+    const uint32_t mod = vals < 32u * 1024u ? vals : 32u * 1024u;
+    const uint32_t nfull = (uint32_t)(((uint64_t)vals + mod - 1) / mod);
+    const uint32_t nblocks = nfull > 1 ? nfull - 1 : 1;
+    double *d_part = nullptr;
+    MDSM_HIP(hipMalloc((void **)&d_part, sizeof(double) * 2 * nblocks));
+    int rc = launch_legacy_block_sums(d_x, mod, nblocks, d_part, s);
+    std::vector<double> h(2 * (size_t)nblocks);
+    if (rc == MDSM_OK) {
+      MDSM_HIP(hipMemcpyAsync(h.data(), d_part, sizeof(double) * 2 * nblocks,
+                              hipMemcpyDeviceToHost, s));
+    }
+    MDSM_HIP(hipStreamSynchronize(s));
+    MDSM_HIP(hipFree(d_part));
+    if (rc != MDSM_OK) return rc;
+    // FP32 running mean of FP64 block means, FP64 mean of squares, as the
+    // reference's variable types round them
+    float localmean = 0.0f;
+    double mean2 = 0.0;
+    for (uint32_t i = 0; i < nblocks; ++i) {
+      localmean = (float)((double)localmean + h[2 * i] / (double)mod);
+      mean2 += h[2 * i + 1] / (double)mod;
+    }
+    localmean = localmean / (float)nblocks;
+    const float lm2 = localmean * localmean;
+    gmean = localmean;
+    grms = (float)std::sqrt(mean2 / (double)nblocks - (double)lm2);
+    if (h_mean) *h_mean = gmean;
+    if (h_rms) *h_rms = grms;
+  }
+  int rc = launch_zero_u32(d_count, 1, s);
+  if (rc != MDSM_OK || nsamp < 5) return rc;
+  return launch_legacy_threshold(d_x, ndms, nsamp, lowdm, dmstep, thresh, gmean, grms, d_mean,
+                                 d_rms, samples_per_chunk, d_cands, cap, d_count, s);
+}
+
 extern "C" int mdsm_median_filter5(const float *d_x, uint32_t nrows, uint32_t nsamp, float *d_out,
                                    void *stream) {
   MDSM_REQUIRE(d_x && d_out && d_x != d_out && nrows <= 65535, MDSM_ERR_ARG,

@@ -256,6 +256,94 @@ __global__ __launch_bounds__(NT) void median5_kernel(const float *__restrict__ x
   out[(uint64_t)d * nsamp + t] = v[2];
 }
 
+// ------------------------------------------------------------ legacy detector
+// process_brute, src/mdsm/dedispersion_output.cpp:135-213.
+//
+// Statistics: consecutive blocks of block_len values are summed in FP64 in
+// index order, one lane per block, because that order fixes the rounding of
+// every block total (the host then folds the block totals in block order).
+// A block is read with 16-byte loads when its start is 16-byte aligned.
+template <bool VEC>
+__global__ __launch_bounds__(64) void legacy_block_sums_kernel(const float *__restrict__ x,
+                                                               uint32_t block_len,
+                                                               uint32_t nblocks,
+                                                               double *__restrict__ sums) {
+  const uint32_t b = blockIdx.x * 64u + threadIdx.x;
+  if (b >= nblocks) return;
+  const float *p = x + (uint64_t)b * block_len;
+  double t = 0.0, t2 = 0.0;
+  if (VEC) {
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+    for (uint32_t j = 0; j < block_len / 4u; ++j) {
+      const float4 v = q[j];
+      const double a = (double)v.x, c = (double)v.y, d = (double)v.z, e = (double)v.w;
+      t = t + a;
+      t2 = t2 + a * a;
+      t = t + c;
+      t2 = t2 + c * c;
+      t = t + d;
+      t2 = t2 + d * d;
+      t = t + e;
+      t2 = t2 + e * e;
+    }
+  } else {
+    for (uint32_t j = 0; j < block_len; ++j) {
+      const double a = (double)p[j];
+      t = t + a;
+      t2 = t2 + a * a;
+    }
+  }
+  sums[2u * b] = t;
+  sums[2u * b + 1u] = t2;
+}
+
+// The selection network of src/mdsm/dedispersion_output.cpp:14-53: it picks
+// one of its five arguments, so the result (signed zeros included) is the one
+// that network returns.
+__device__ __forceinline__ float median5_network(float n1, float n2, float n3, float n4,
+                                                 float n5) {
+  float a = n1, b = n2, c = n3, d = n4, t;
+  if (b < a) { t = a; a = b; b = t; }
+  if (d < c) { t = c; c = d; d = t; }
+  if (c < a) { t = b; b = d; d = t; c = a; }
+  a = n5;
+  if (b < a) { t = a; a = b; b = t; }
+  if (a < c) { t = b; b = d; d = t; a = c; }
+  return d < a ? d : a;
+}
+
+// Median of five over samples l..l+4 of every row, for l < nsamp - 4, then
+// the threshold in FP32 as written (no contraction): with per-chunk
+// statistics (cmean != nullptr) against cmean/crms[l / spc] with snr =
+// (median - mean) / rms, else against the global gmean/grms with snr =
+// median / grms; only rows whose DM lowdm + k*dmstep exceeds 1 report.
+__global__ __launch_bounds__(NT) void legacy_threshold_kernel(
+    const float *__restrict__ x, uint32_t nsamp, float lowdm, float dmstep, float thresh,
+    float gmean, float grms, const float *__restrict__ cmean, const float *__restrict__ crms,
+    uint32_t spc, mdsm_cand_t *__restrict__ cands, uint32_t cap, uint32_t *__restrict__ count) {
+  const uint32_t k = blockIdx.y;
+  const uint32_t l = blockIdx.x * NT + threadIdx.x;
+  const float *r = x + (uint64_t)k * nsamp;
+  const float thisdm = __fadd_rn(lowdm, __fmul_rn((float)k, dmstep));
+  bool hit = false;
+  float snr = 0.0f;
+  if (l + 4u < nsamp && thisdm > 1.0f) {
+    const float med = median5_network(r[l], r[l + 1], r[l + 2], r[l + 3], r[l + 4]);
+    if (cmean) {
+      const uint32_t i = l / spc;
+      const float m = cmean[i], sd = crms[i];
+      const float dev = __fsub_rn(med, m);
+      hit = dev >= __fmul_rn(sd, thresh);
+      snr = __fdiv_rn(dev, sd);
+    } else {
+      hit = __fsub_rn(med, gmean) >= __fmul_rn(grms, thresh);
+      snr = __fdiv_rn(med, grms);
+    }
+  }
+  mdsm_cand_t cand{(int32_t)k, (int32_t)l, 0, snr};
+  emit(hit, cand, cands, cap, count);
+}
+
 // ---------------------------------------------------------------- detrend
 __global__ __launch_bounds__(NT) void detrend_kernel(float *__restrict__ x, uint32_t L) {
   __shared__ double p1[NT], p2[NT];
@@ -429,6 +517,31 @@ int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t s) {
 
 int launch_median5(const float *x, uint32_t nrows, uint32_t nsamp, float *out, hipStream_t s) {
   hipLaunchKernelGGL(median5_kernel, dim3((nsamp + NT - 1) / NT, nrows), dim3(NT), 0, s, x, nsamp, out);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_legacy_block_sums(const float *x, uint32_t block_len, uint32_t nblocks, double *sums,
+                             hipStream_t s) {
+  const bool vec = (block_len % 4u) == 0u && (reinterpret_cast<uintptr_t>(x) & 15u) == 0u;
+  const dim3 grid((nblocks + 63u) / 64u);
+  if (vec)
+    hipLaunchKernelGGL(legacy_block_sums_kernel<true>, grid, dim3(64), 0, s, x, block_len, nblocks,
+                       sums);
+  else
+    hipLaunchKernelGGL(legacy_block_sums_kernel<false>, grid, dim3(64), 0, s, x, block_len,
+                       nblocks, sums);
+  MDSM_HIP(hipGetLastError());
+  return MDSM_OK;
+}
+
+int launch_legacy_threshold(const float *x, uint32_t ndms, uint32_t nsamp, float lowdm,
+                            float dmstep, float thresh, float gmean, float grms, const float *cmean,
+                            const float *crms, uint32_t spc, mdsm_cand_t *cands, uint32_t cap,
+                            uint32_t *count, hipStream_t s) {
+  hipLaunchKernelGGL(legacy_threshold_kernel, dim3((nsamp - 4u + NT - 1) / NT, ndms), dim3(NT), 0,
+                     s, x, nsamp, lowdm, dmstep, thresh, gmean, grms, cmean, crms, spc, cands, cap,
+                     count);
   MDSM_HIP(hipGetLastError());
   return MDSM_OK;
 }

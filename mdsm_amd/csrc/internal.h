@@ -115,6 +115,12 @@ int launch_threshold_global(const float *x, uint32_t nrows, uint32_t nsamp, doub
                             uint32_t *count, hipStream_t s);
 int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t s);
 int launch_median5(const float *x, uint32_t nrows, uint32_t nsamp, float *out, hipStream_t s);
+int launch_legacy_block_sums(const float *x, uint32_t block_len, uint32_t nblocks, double *sums,
+                             hipStream_t s);
+int launch_legacy_threshold(const float *x, uint32_t ndms, uint32_t nsamp, float lowdm,
+                            float dmstep, float thresh, float gmean, float grms, const float *cmean,
+                            const float *crms, uint32_t spc, mdsm_cand_t *cands, uint32_t cap,
+                            uint32_t *count, hipStream_t s);
 int launch_detrend(float *x, uint64_t nseg, uint32_t L, hipStream_t s);
 int launch_unpack_corner_turn(const void *in, int nbits, uint32_t nspec, uint32_t nchans,
                               uint32_t nbeams, float *out, uint64_t ld, uint64_t beam_stride,

@@ -85,6 +85,8 @@ _sig("oracle_median5", _f, _f, _f, _f, _f, _f)
 _sig("oracle_median_filter5", None, _f32p, _i, _i, _f32p)
 _sig("oracle_process_brute_legacy", _i64, _f32p, _i, _i, _f, _f, _f, _i64, _i32p, _i32p,
      _f32p, C.POINTER(_f), C.POINTER(_f))
+_sig("oracle_process_brute_legacy_chunks", _i64, _f32p, _i, _i, _f, _f, _f, _f32p, _f32p, _i,
+     _i64, _i32p, _i32p, _f32p)
 _sig("oracle_detrend_normalise", None, _f32p, _i64, _i, _f32p)
 _sig("oracle_num_threads", _i)
 _sig("oracle_sigproc_unpack", None, _u8p, _i, _i64, _f32p)
@@ -344,6 +346,22 @@ def process_brute_legacy(x, lowdm, dmstep, thresh, cap=1 << 20):
                                          C.byref(m), C.byref(r))
     k = min(n, cap)
     return cd[:k].copy(), ct[:k].copy(), cs[:k].copy(), int(n), m.value, r.value
+
+
+def process_brute_legacy_chunks(x, lowdm, dmstep, thresh, mean, rms, spc, cap=1 << 20):
+    """Legacy detector with caller-supplied per-chunk mean and rms."""
+    x = _c(x, np.float32)
+    ndms, nsamp = x.shape
+    mean = _c(mean, np.float32)
+    rms = _c(rms, np.float32)
+    assert spc > 0 and mean.size >= max(nsamp - 5, 0) // spc + 1 and rms.size == mean.size
+    cd = np.empty(cap, np.int32)
+    ct = np.empty(cap, np.int32)
+    cs = np.empty(cap, np.float32)
+    n = _lib.oracle_process_brute_legacy_chunks(x, ndms, nsamp, lowdm, dmstep, thresh, mean, rms,
+                                                spc, cap, cd, ct, cs)
+    k = min(n, cap)
+    return cd[:k].copy(), ct[:k].copy(), cs[:k].copy(), int(n)
 
 
 def detrend_normalise(x, L):

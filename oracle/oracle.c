@@ -774,6 +774,30 @@ int64_t oracle_process_brute_legacy(const float *x, int ndms, int nsamp, float l
   return total_c;
 }
 
+/* Legacy detector, branch with caller-supplied statistics
+ * (dedispersion_output.cpp:198-209, mean != NULL): sample l of every row is
+ * judged against mean[l / spc] and rms[l / spc] (per-chunk noise estimates),
+ * snr = (median - mean) / rms.  Same median of five and DM > 1 rule. */
+int64_t oracle_process_brute_legacy_chunks(const float *x, int ndms, int nsamp, float lowdm,
+                                           float dmstep, float thresh, const float *mean,
+                                           const float *rms, int spc, int64_t cap, int32_t *cd,
+                                           int32_t *ct, float *csnr) {
+  int64_t total_c = 0;
+  for (int k = 0; k < ndms; ++k) {
+    const float *r = x + (int64_t)k * nsamp;
+    for (int l = 0; l < nsamp - 4; ++l) {
+      float med = oracle_median5(r[l], r[l + 1], r[l + 2], r[l + 3], r[l + 4]);
+      float thisdm = lowdm + k * dmstep;
+      float m = mean[l / spc], s = rms[l / spc];
+      if (med - m >= s * thresh && thisdm > 1.0) {
+        if (total_c < cap) { cd[total_c] = k; ct[total_c] = l; csnr[total_c] = (med - m) / s; }
+        total_c++;
+      }
+    }
+  }
+  return total_c;
+}
+
 /* Linear detrend and normalisation of consecutive segments of length L
  * (detrend_normalise, multibeam_dedispersion_kernel.cu:207-295), FP64
  * accumulation: x_i = -L/2 + 0.5 + i; fit y = a + b x by least squares

@@ -140,6 +140,87 @@ def test_detect_global_reference_semantics():
     assert list(zip(c["dm_idx"], c["sample"])) == list(zip(od, ot))
 
 
+def _legacy_gpu(x, lowdm, dmstep, thresh, mean=None, rms=None, spc=0, offset=0):
+    ndms, nsamp = x.shape
+    # offset shifts the device copy by whole floats (16-byte misalignment)
+    d_x = abi.DeviceBuffer.from_numpy(np.concatenate([np.zeros(offset, np.float32), x.ravel()]))
+    cap = 1 << 18
+    d_c, d_n = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+    d_m = abi.DeviceBuffer.from_numpy(mean) if mean is not None else None
+    d_r = abi.DeviceBuffer.from_numpy(rms) if rms is not None else None
+    st = abi.detect_legacy(d_x.ptr + 4 * offset, ndms, nsamp, lowdm, dmstep, thresh, d_c.ptr, cap,
+                           d_n.ptr, d_m.ptr if d_m else None, d_r.ptr if d_r else None, spc,
+                           0 if mean is None else mean.size)
+    return st, abi.read_candidates(d_c, d_n, cap)
+
+
+@pytest.mark.parametrize("case", ["pulse", "odd_small", "quirk", "misaligned", "short_rows"])
+def test_detect_legacy_bitexact(case):
+    """Global-statistics branch of the legacy detector: mean and rms equal
+    the oracle's FP32 values exactly, candidates (row, sample, snr) too."""
+    rng = np.random.default_rng(21)
+    off = 0
+    if case == "pulse":
+        _, _, _, x = _dedispersed(13, 64, 48, 20000, pulses=[(2.0, 3000, 4, 1.0)])
+        args = (0.0, 0.1, 3.0)
+    elif case == "odd_small":
+        x = rng.standard_normal((3, 1001)).astype(np.float32)
+        args = (0.3, 0.35, 1.5)
+    elif case == "quirk":
+        x = np.zeros((3, 32768), np.float32)
+        x[0], x[1], x[2] = 1.0, 3.0, 100.0
+        args = (0.0, 2.0, 6.0)
+    elif case == "misaligned":
+        x = (rng.standard_normal((9, 32771)) * 3 + 1).astype(np.float32)
+        args = (0.5, 0.25, 2.5)
+        off = 1
+    else:
+        x = rng.standard_normal((4, 5)).astype(np.float32) * 4
+        args = (1.5, 1.0, 0.0)
+    _say("legacy", case, x.shape)
+    (m, r), c = _legacy_gpu(x, *args, offset=off)
+    od, ot, osnr, on, om, orms = O.process_brute_legacy(x, *args)
+    assert (m, r) == (om, orms), (m, r, om, orms)
+    assert len(c) == on
+    assert c["dm_idx"].tolist() == od.tolist() and c["sample"].tolist() == ot.tolist()
+    assert np.array_equal(c["snr"], osnr)
+    if case == "quirk":
+        assert (m, r) == (2.0, 1.0) and on > 0
+
+
+def test_detect_legacy_chunk_statistics_bitexact():
+    rng = np.random.default_rng(22)
+    ndms, nsamp, spc = 40, 12345, 1000
+    x = rng.standard_normal((ndms, nsamp)).astype(np.float32)
+    x[:, 5000:5010] += 6.0
+    nstats = (nsamp - 5) // spc + 1
+    mean = (rng.random(nstats) * 0.4 - 0.2).astype(np.float32)
+    rms = (0.8 + rng.random(nstats) * 0.4).astype(np.float32)
+    st, c = _legacy_gpu(x, 0.2, 0.05, 2.0, mean, rms, spc)
+    assert st is None
+    od, ot, osnr, on = O.process_brute_legacy_chunks(x, 0.2, 0.05, 2.0, mean, rms, spc)
+    assert on > 100 and len(c) == on
+    assert c["dm_idx"].tolist() == od.tolist() and c["sample"].tolist() == ot.tolist()
+    assert np.array_equal(c["snr"], osnr)
+
+
+def test_detect_legacy_argument_errors():
+    x = np.zeros((4, 100), np.float32)
+    d_x = abi.DeviceBuffer.from_numpy(x)
+    d_c, d_n = abi.DeviceBuffer(16 * 16), abi.DeviceBuffer(4)
+    d_m = abi.DeviceBuffer.from_numpy(np.zeros(2, np.float32))
+    with pytest.raises(abi.MdsmError):   # mean without rms
+        abi.detect_legacy(d_x.ptr, 4, 100, 0.0, 1.0, 3.0, d_c.ptr, 16, d_n.ptr, d_m.ptr, None, 50, 2)
+    with pytest.raises(abi.MdsmError):   # (100 - 5) // 40 + 1 = 3 entries needed
+        abi.detect_legacy(d_x.ptr, 4, 100, 0.0, 1.0, 3.0, d_c.ptr, 16, d_n.ptr, d_m.ptr, d_m.ptr,
+                          40, 2)
+    with pytest.raises(abi.MdsmError):   # samples_per_chunk 0
+        abi.detect_legacy(d_x.ptr, 4, 100, 0.0, 1.0, 3.0, d_c.ptr, 16, d_n.ptr, d_m.ptr, d_m.ptr,
+                          0, 2)
+    with pytest.raises(abi.MdsmError):   # the value count must fit 32 bits
+        abi.detect_legacy(d_x.ptr, 65535, 65538, 0.0, 1.0, 3.0, d_c.ptr, 16, d_n.ptr)
+
+
 def test_median_and_detrend():
     rng = np.random.default_rng(14)
     x = rng.standard_normal((7, 3001)).astype(np.float32)

@@ -186,6 +186,29 @@ def test_legacy_detector_block_mean_quirk():
     assert d.tolist() == [1, 1, 1] and t.tolist() == [498, 499, 500]
 
 
+def test_legacy_detector_chunk_statistics():
+    # per-chunk statistics: the same 3-sample pulse passes in chunk 0 (mean 0,
+    # rms 1) and fails in chunk 1 (mean 40); snr = (median - mean) / rms
+    y = np.zeros((3, 400), np.float32)
+    y[:, 100:103] = 50.0
+    y[:, 300:303] = 50.0
+    mean = np.array([0.0, 0.0, 40.0, 40.0], np.float32)
+    rms = np.array([1.0, 1.0, 2.0, 2.0], np.float32)
+    d, t, s, n = O.process_brute_legacy_chunks(y, 0.0, 0.75, 6.0, mean, rms, 100)
+    # rows 0 and 1 have DM 0 and 0.75 (not above 1); row 2 has DM 1.5
+    assert n == 3 and d.tolist() == [2, 2, 2] and t.tolist() == [98, 99, 100]
+    assert s.tolist() == [50.0, 50.0, 50.0]
+    # with threshold 4 the second pulse passes too: (50 - 40) / 2 = 5 >= 4
+    d, t, s, n = O.process_brute_legacy_chunks(y, 0.0, 0.75, 4.0, mean, rms, 100)
+    assert t.tolist() == [98, 99, 100, 298, 299, 300] and s.tolist()[3:] == [5.0, 5.0, 5.0]
+    # the chunk index is l / spc on the median's first sample: l = 199 sits in
+    # chunk 1 (mean 0), l = 200 in chunk 2 (mean 40)
+    z = np.zeros((2, 400), np.float32)
+    z[1, 199:204] = 30.0
+    d, t, s, n = O.process_brute_legacy_chunks(z, 0.0, 2.0, 6.0, mean, rms, 100)
+    assert t.tolist() == [197, 198, 199]
+
+
 def test_detrend_vs_polyfit():
     rng = np.random.default_rng(6)
     L = 1024
