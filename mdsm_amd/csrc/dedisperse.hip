@@ -26,7 +26,8 @@
 //        the exact sum in cancellation and wide-exponent cases.)
 //   U8   2 B per add: bytes widened to pairs of u16 in the same two-copy
 //        layout, summed exactly with v_pk_add_u16 (two adds per
-//        instruction) and folded into u32 every <= 257 channels.
+//        instruction); every <= 256 channels their high bytes move into a
+//        second u16 pair accumulator (sum = 256 * high + low, exact).
 // Results are bit-identical to a sequential FP32 sum in channel order (F32,
 // F16 storage) or to the exact integer sum (U8).  MFMA is not used: the
 // operation is shift-and-add, not a contraction.
@@ -378,17 +379,21 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
       }
     };
 
-    // F16: FP32 accumulators, U8: u16 pair accumulators folded into u32
+    // F16: FP32 accumulators.  U8: two u16 pair accumulators per output
+    // pair, the running sum S = 256*uhi + uacc held exactly: a fold moves
+    // uacc's high bytes into uhi (three packed instructions per pair) and
+    // leaves uacc <= 255, so 256 more channels of bytes fit in 16 bits, and
+    // uhi <= S/256 < nchans fits while nchans <= 65536.
     float facc[IS_U8 ? 1 : MW][IS_U8 ? 1 : K2][2];
     ushort2_t uacc[IS_U8 ? MW : 1][IS_U8 ? K2 : 1];
-    uint32_t usum[IS_U8 ? MW : 1][IS_U8 ? K2 : 1][2];
+    ushort2_t uhi[IS_U8 ? MW : 1][IS_U8 ? K2 : 1];
 #pragma unroll
     for (int i = 0; i < MW; ++i)
 #pragma unroll
       for (int j = 0; j < K2; ++j) {
         if (IS_U8) {
           uacc[i][j] = ushort2_t{0, 0};
-          usum[i][j][0] = usum[i][j][1] = 0u;
+          uhi[i][j] = ushort2_t{0, 0};
         } else {
           facc[i][j][0] = facc[i][j][1] = 0.0f;
         }
@@ -399,9 +404,8 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
         for (int i = 0; i < MW; ++i)
 #pragma unroll
           for (int j = 0; j < K2; ++j) {
-            usum[i][j][0] += uacc[i][j].x;
-            usum[i][j][1] += uacc[i][j].y;
-            uacc[i][j] = ushort2_t{0, 0};
+            uhi[i][j] += uacc[i][j] >> (ushort)8;
+            uacc[i][j] &= (ushort)0xFF;
           }
       }
     };
@@ -430,7 +434,7 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
       }
       const uint32_t nc = min(cbr, nchans - cb);
       if (IS_U8) {
-        if (since_fold + nc > 257u) {  // 257 * 255 = 65535 still fits in 16 bits
+        if (since_fold + nc > 256u) {  // 255 + 256 * 255 = 65535 still fits in 16 bits
           fold();
           since_fold = 0;
         }
@@ -482,7 +486,6 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
         rel = rel_next;
       }
     }
-    fold();
 
     const double base = (double)nchans * -4.0;
 #pragma unroll
@@ -495,8 +498,10 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
           const uint32_t t = t0 + 2u * lane + 128u * j;
           float v0, v1;
           if (IS_U8) {
-            v0 = (float)((double)usum[i][j][0] * (8.0 / 255.0) + base);
-            v1 = (float)((double)usum[i][j][1] * (8.0 / 255.0) + base);
+            const uint32_t s0 = 256u * (uint32_t)uhi[i][j].x + (uint32_t)uacc[i][j].x;
+            const uint32_t s1 = 256u * (uint32_t)uhi[i][j].y + (uint32_t)uacc[i][j].y;
+            v0 = (float)((double)s0 * (8.0 / 255.0) + base);
+            v1 = (float)((double)s1 * (8.0 / 255.0) + base);
           } else {
             v0 = facc[i][j][0];
             v1 = facc[i][j][1];

@@ -103,7 +103,7 @@ def test_f16_u8_bitexact_and_error():
 def test_packed_ragged_shapes_all_variants(shape):
     """F16 and U8 storage, every tile variant, on shapes that leave partial
     channel blocks, DM tiles and time tiles (odd channel counts for the pair
-    copies; 600 channels cross the U8 16-bit fold at 257)."""
+    copies; 600 channels cross the U8 16-bit fold every 256 channels)."""
     C, M, N, lo, step = shape
     s = _survey(N, nchans=C, ndms=M, lowdm=lo, dmstep=step, foff=-0.078125 * 256 / C / 4)
     t = _table(s)
@@ -116,6 +116,21 @@ def test_packed_ragged_shapes_all_variants(shape):
     ref8 = O.dedisperse_u8(u, t, N)
     for v in range(1, abi.tile_variants() + 1):
         assert np.array_equal(abi.dedisperse_host(s, h, abi.F16, v), ref16), ("f16", v)
+        assert np.array_equal(abi.dedisperse_host(s, u, abi.U8, v), ref8), ("u8", v)
+
+
+def test_u8_near_saturated_sums_all_variants():
+    """Bytes of 250..255 over 1500 channels: sums up to 382500 need the
+    high-part accumulator after every 256-channel window, in every variant."""
+    C, M, N = 1500, 40, 600
+    s = _survey(N, nchans=C, ndms=M, lowdm=0.0, dmstep=0.5, foff=-0.078125 * 256 / C / 4)
+    t = _table(s)
+    S = int(t.max())
+    _say(f"[u8 saturated] C={C} M={M} N={N} S={S}")
+    u = np.random.default_rng(31).integers(250, 256, (C, N + S + 4)).astype(np.uint8)
+    u[:, ::7] = 255
+    ref8 = O.dedisperse_u8(u, t, N)
+    for v in range(1, abi.tile_variants() + 1):
         assert np.array_equal(abi.dedisperse_host(s, u, abi.U8, v), ref8), ("u8", v)
 
 
