@@ -54,11 +54,16 @@
  *                      headers: unbuildable here; chained neighbourhoods are
  *                      pinned only by the hand cases)
  *   classify_ref       hand-derived cases (RFI and width rules, the
- *                      histogram smoothing and median, the model curve's
- *                      closed form at its peak); parity with the reference
- *                      source is unpinned beyond them (it needs PGPLOT and
- *                      Qt headers)
+ *                      histogram smoothing and median), and the whole score
+ *                      on clusters whose smoothed DM-S/N curve is the
+ *                      Cordes & McLaughlin (2003) smearing response: 0 when
+ *                      only the peak bin passes the 0.9 cut, (1/m-1)^2
+ *                      sum(f^2)/N otherwise (the reference source needs
+ *                      PGPLOT and Qt headers: not compiled here)
  *   median5            sorted median over random and exhaustive-order inputs
+ *   legacy detector    the block-count quirk (last block left out), median
+ *                      spikes and plateaus, DM > 1 rule, hand-derived
+ *                      per-chunk statistics cases
  *   f16 conversion     numpy float16 (round to nearest even)
  *   bandpass / RFI     exact sums on integer data, float64 means within
  *                      float rounding, hand-built bursts and spikes whose
