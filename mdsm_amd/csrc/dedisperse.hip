@@ -696,7 +696,10 @@ template <> struct VariantDef<12> { using F32 = F32Tile<12, 6, 4>;  using F16 = 
 template <> struct VariantDef<13> { using F32 = F32Tile<16, 6, 4>;  using F16 = F16Tile<16, 3, 4>;  using U8 = U8Tile<16, 3, 4>; };
 // 16 trials per wave over 8-channel blocks: half the barriers per add
 template <> struct VariantDef<14> { using F32 = F32Tile<16, 4, 8>;  using F16 = F16Tile<16, 2, 8>;  using U8 = U8Tile<16, 2, 8>; };
-constexpr int kNumVariants = 14;
+// 12 trials per wave, 2 pairs per lane: fewer accumulators, 7 waves per
+// SIMD for the packed tiles (68-72 VGPRs)
+template <> struct VariantDef<15> { using F32 = F32Tile<12, 4, 4>;  using F16 = F16Tile<12, 2, 4>;  using U8 = U8Tile<12, 2, 4>; };
+constexpr int kNumVariants = 15;
 
 template <typename V, typename Fn>
 int pick_dtype(int dtype, Fn &&fn) {
@@ -726,6 +729,7 @@ int with_tile(int dtype, int variant, Fn &&fn) {
     case 12: return pick_dtype<VariantDef<12>>(dtype, fn);
     case 13: return pick_dtype<VariantDef<13>>(dtype, fn);
     case 14: return pick_dtype<VariantDef<14>>(dtype, fn);
+    case 15: return pick_dtype<VariantDef<15>>(dtype, fn);
   }
   set_error("unknown dedispersion variant %d", variant);
   return MDSM_ERR_ARG;
@@ -746,7 +750,7 @@ uint32_t stage_channels(uint32_t rw) {
 
 int num_variants(int) { return kNumVariants; }
 
-int preferred_variant(int dtype) { return dtype == MDSM_F32 ? 8 : 2; }
+int preferred_variant(int dtype) { return dtype == MDSM_F32 ? 8 : (dtype == MDSM_F16 ? 15 : 2); }
 
 TileShape variant_shape(int dtype, int variant) {
   TileShape sh{0, 0, 0};
