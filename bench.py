@@ -659,7 +659,12 @@ def main():
     if os.path.exists(TRAFFIC_FILE):  # measured HBM bytes per launch (rocprofv3 --pmc)
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        traffic = tr.get("bytes_per_launch")
+        # counters of another tile shape do not describe this launch
+        if tr.get("tile_variant") == v16:
+            traffic = tr.get("bytes_per_launch")
+        else:
+            log(f"traffic file profiled tile variant {tr.get('tile_variant')}, this run chose {v16}: "
+                "roofline.traffic left null")
     line = {
         "metric": "dedispersed DM x samples per second (batched, FP16 storage)",
         "value": value,
