@@ -46,8 +46,12 @@ enum { MDSM_F32 = 0, MDSM_F16 = 1, MDSM_U8 = 2 };
  * REFERENCE: (int)(dm_shifts[c]*(lowdm + d*dmstep)/tsamp) in float with
  *   dm_shifts[c] = (float)(4148.741601*(F1^-2 - F2^-2)), F1 = fch1 + foff*c in
  *   float (multibeam_dedispersion_manager.cpp:325-326,366-369;
- *   multibeam_dedispersion_thread.cu:551-553). */
-enum { MDSM_DELAY_FP64_ROUND = 0, MDSM_DELAY_REFERENCE = 1 };
+ *   multibeam_dedispersion_thread.cu:551-553).
+ * LEGACY: floor(dm_shifts[c]*((lowdm + d*dmstep)/tsamp)) in float, the same
+ *   dm_shifts (the single-beam manager's brute force:
+ *   src/mdsm/dedispersion_manager.cpp:262-265,284-286 and
+ *   src/mdsm/dedispersion_kernel.cu:55,68-69). */
+enum { MDSM_DELAY_FP64_ROUND = 0, MDSM_DELAY_REFERENCE = 1, MDSM_DELAY_LEGACY = 2 };
 
 /* Survey parameters of one beam: the <observation> XML fields of the
  * reference (SURVEY/BEAM, src/mdsm-multibeam/survey.h:8-88). */

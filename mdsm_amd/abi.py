@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libmdsm_amd.so")
 (MDSM_OK, MDSM_ERR_ARG, MDSM_ERR_HIP, MDSM_ERR_NOMEM, MDSM_ERR_OVERFLOW, MDSM_ERR_STATE,
  MDSM_ERR_IO, MDSM_ERR_FORMAT) = range(8)
 F32, F16, U8 = 0, 1, 2
-DELAY_FP64_ROUND, DELAY_REFERENCE = 0, 1
+DELAY_FP64_ROUND, DELAY_REFERENCE, DELAY_LEGACY = 0, 1, 2
 _ITEMSIZE = {F32: 4, F16: 2, U8: 1}
 _NP = {F32: np.float32, F16: np.uint16, U8: np.uint8}
 ALIGN = {F32: 1, F16: 2, U8: 4}  # required multiple of the input row length

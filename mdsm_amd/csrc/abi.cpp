@@ -54,19 +54,26 @@ int build_delay_table(const mdsm_survey_t *s, std::vector<int32_t> &table, uint3
       for (uint32_t d = 0; d < s->ndms; ++d)
         table[(uint64_t)c * s->ndms + d] =
             delay_ns(s->fch1, s->foff, c, s->tsamp, s->lowdm + s->dmstep * (double)d);
-  } else if (s->delay_mode == MDSM_DELAY_REFERENCE) {
-    // float dm_shifts (manager.cpp:366-369) and the truncating product
-    // (thread.cu:553), float arithmetic throughout.
+  } else if (s->delay_mode == MDSM_DELAY_REFERENCE || s->delay_mode == MDSM_DELAY_LEGACY) {
+    // float dm_shifts (manager.cpp:366-369), float arithmetic throughout.
+    // REFERENCE: the truncated (shift * dm) / tsamp of thread.cu:553;
+    // LEGACY: floor(shift * (dm / tsamp)) of src/mdsm/dedispersion_kernel.cu:55,68.
     const float fch1 = (float)s->fch1, foff = (float)s->foff, tsamp = (float)s->tsamp;
     const float lowdm = (float)s->lowdm, dmstep = (float)s->dmstep;
+    const bool legacy = s->delay_mode == MDSM_DELAY_LEGACY;
     for (uint32_t c = 0; c < s->nchans; ++c) {
       const float F1 = fch1 + foff * (float)c;
       const float sh = (float)(4148.741601 * ((1.0 / F1 / F1) - (1.0 / fch1 / fch1)));
       for (uint32_t d = 0; d < s->ndms; ++d) {
         const float dm = lowdm + (float)d * dmstep;
-        float v = sh * dm;
-        v = v / tsamp;
-        table[(uint64_t)c * s->ndms + d] = (int32_t)v;
+        if (legacy) {
+          const float per_sample = dm / tsamp;
+          table[(uint64_t)c * s->ndms + d] = (int32_t)std::floor(sh * per_sample);
+        } else {
+          float v = sh * dm;
+          v = v / tsamp;
+          table[(uint64_t)c * s->ndms + d] = (int32_t)v;
+        }
       }
     }
   } else {

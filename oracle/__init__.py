@@ -61,6 +61,7 @@ _sig("oracle_generate", None, _u64, _i, _d, _d, _d, _i64, _i64, _f32p, _f32p, _i
 _sig("oracle_delay_table", None, _d, _d, _i, _d, _d, _d, _i, _i32p)
 _sig("oracle_dm_shifts_ref", None, _f, _f, _i, _f32p)
 _sig("oracle_delay_table_ref", None, _f, _f, _i, _f, _f, _f, _i, _i32p)
+_sig("oracle_delay_table_legacy", None, _f, _f, _i, _f, _f, _f, _i, _i32p)
 _sig("oracle_maxshift_ref", _u32, _f32p, _f32p, _i, _i, _f, _f, _f, _i)
 _sig("oracle_shared_shift_ref", _u32, _f, _f, _f, _i, _f, _i)
 _sig("oracle_calculate_nsamp_ref", _u32, _u32, _u32, _u32, _u32, _u64)
@@ -165,6 +166,12 @@ def dm_shifts_ref(fch1, foff, nchans):
 def delay_table_ref(fch1, foff, nchans, tsamp, lowdm, dmstep, ndms):
     t = np.empty((nchans, ndms), np.int32)
     _lib.oracle_delay_table_ref(fch1, foff, nchans, tsamp, lowdm, dmstep, ndms, t)
+    return t
+
+
+def delay_table_legacy(fch1, foff, nchans, tsamp, lowdm, dmstep, ndms):
+    t = np.empty((nchans, ndms), np.int32)
+    _lib.oracle_delay_table_legacy(fch1, foff, nchans, tsamp, lowdm, dmstep, ndms, t)
     return t
 
 

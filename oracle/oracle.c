@@ -227,6 +227,20 @@ void oracle_delay_table_ref(float fch1, float foff, int nchans, float tsamp, flo
   free(s);
 }
 
+/* Legacy single-beam brute-force delays (src/mdsm/dedispersion_kernel.cu:55,68):
+ * shift_temp = (startdm + d*dmstep)/tsamp, delay = floor(dm_shifts[c]*shift_temp),
+ * float throughout, with the same dm_shifts (src/mdsm/dedispersion_manager.cpp:262-265). */
+void oracle_delay_table_legacy(float fch1, float foff, int nchans, float tsamp, float lowdm,
+                               float dmstep, int ndms, int32_t *table) {
+  float *s = (float *)malloc(sizeof(float) * (size_t)nchans);
+  oracle_dm_shifts_ref(fch1, foff, nchans, s);
+  for (int d = 0; d < ndms; ++d) {
+    float shift_temp = (lowdm + (float)d * dmstep) / tsamp;
+    for (int c = 0; c < nchans; ++c) table[(int64_t)c * ndms + d] = (int32_t)floorf(s[c] * shift_temp);
+  }
+  free(s);
+}
+
 /* Reference maxshift for a list of beams (manager.cpp:362-380), including the
  * round-up to a multiple of 256 that is applied once per beam inside the loop. */
 uint32_t oracle_maxshift_ref(const float *fch1, const float *foff, int nbeams, int nchans,

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-@pytest.mark.parametrize("mode", [abi.DELAY_FP64_ROUND, abi.DELAY_REFERENCE])
+@pytest.mark.parametrize("mode", [abi.DELAY_FP64_ROUND, abi.DELAY_REFERENCE, abi.DELAY_LEGACY])
 @pytest.mark.parametrize("cfg", [
     (418.0, -0.078125, 256, 51.2e-6, 0.0, 0.1, 256),
     (1500.0, -0.29296875, 1024, 64e-6, 0.0, 0.2, 4096),
@@ -36,7 +36,8 @@ def test_library_exports_every_declared_symbol():
 def test_delay_table_matches_oracle(cfg, mode):
     fch1, foff, C, ts, lo, st, M = cfg
     t, ms = abi.delay_table(abi.Survey.make(fch1, foff, C, ts, lo, st, M, 1024, mode))
-    ref = (O.delay_table if mode == abi.DELAY_FP64_ROUND else O.delay_table_ref)(*cfg)
+    ref = {abi.DELAY_FP64_ROUND: O.delay_table, abi.DELAY_REFERENCE: O.delay_table_ref,
+           abi.DELAY_LEGACY: O.delay_table_legacy}[mode](*cfg)
     assert np.array_equal(t, ref)
     assert ms == ref.max()
 

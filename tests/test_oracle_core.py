@@ -88,6 +88,35 @@ def test_delay_table_matches_high_precision(cfg):
             assert t[c, d] == int(v.to_integral_value(rounding="ROUND_HALF_UP")), (c, d)
 
 
+@pytest.mark.parametrize("cfg", [
+    (418.0, -0.078125, 256, 51.2e-6, 0.0, 0.1, 256),
+    (1500.0, -0.29296875, 1024, 64e-6, 0.0, 0.2, 4096),
+    (1400.0, -0.5, 512, 100e-6, 3.0, 0.37, 999),
+])
+def test_legacy_delay_table_is_floor_of_the_float_shift(cfg):
+    """The legacy table is floor(dm_shifts[c] * DM / tsamp): against the
+    exact value of that product (the float dm_shifts and DM taken as given)
+    it may differ only where float rounding reaches an integer boundary; and
+    its operation order (DM / tsamp first) differs from the multibeam one on
+    some cells of a large table."""
+    fch1, foff, C, ts, lo, step, M = cfg
+    t = O.delay_table_legacy(*cfg)
+    r = O.delay_table_ref(*cfg)
+    sh = O.dm_shifts_ref(fch1, foff, C)
+    assert (t[0, :] == 0).all() and (np.diff(t, axis=0) >= 0).all() and (np.diff(t, axis=1) >= 0).all()
+    rng = np.random.default_rng(1)
+    ts32 = Decimal(float(np.float32(ts)))
+    for c, d in zip(rng.integers(0, C, 400), rng.integers(0, M, 400)):
+        dm = np.float32(np.float32(lo) + np.float32(d) * np.float32(step))
+        v = Decimal(float(sh[c])) * Decimal(float(dm)) / ts32
+        near = abs(v - v.to_integral_value()) <= v * Decimal("1e-6") + Decimal("1e-6")
+        if not near:
+            assert t[c, d] == int(v), (c, d, v)
+    assert abs(t.astype(np.int64) - r).max() <= 1
+    if C * M >= 1 << 20:
+        assert (t != r).any()
+
+
 def test_survey_maxshift_values():
     # maxshift S = D[C-1][M-1] at the two single-beam workloads
     t0 = O.delay_table(418.0, -0.078125, 256, 51.2e-6, 0.0, 0.1, 256)
