@@ -188,9 +188,11 @@ int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, float t
  * samples l..l+4 (the selection network of dedispersion_output.cpp:14-53) is
  * a candidate iff DM > 1 and
  *   d_mean == NULL: median - mean >= rms*thresh, snr = median / rms, where
- *     mean and rms come from FP64 sums over 32768-value blocks in the
- *     reference's order and types (its last block is left out when there is
- *     more than one); h_mean/h_rms receive them;
+ *     mean and rms come from FP64 sums over 32768-value blocks of the first
+ *     nvals values of the row-major array (the reference passes
+ *     read_nsamp*ndms; 0 means ndms*nsamp) in the reference's order and
+ *     types (its last block is left out when there is more than one);
+ *     h_mean/h_rms receive them;
  *   d_mean != NULL: median - d_mean[l/spc] >= d_rms[l/spc]*thresh, snr =
  *     (median - d_mean[l/spc]) / d_rms[l/spc], with nstats entries in each
  *     device array (at least (nsamp-5)/spc + 1); h_mean/h_rms untouched.
@@ -198,10 +200,26 @@ int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, float t
  * *d_count receives the number of candidates (may exceed cap; order not
  * defined). */
 int mdsm_detect_legacy(const float *d_x, uint32_t ndms, uint32_t nsamp, float lowdm,
-                       float dmstep, float thresh, const float *d_mean, const float *d_rms,
+                       float dmstep, float thresh, uint64_t nvals, const float *d_mean,
+                       const float *d_rms,
                        uint32_t samples_per_chunk, uint32_t nstats, mdsm_cand_t *d_cands,
                        uint32_t cap, uint32_t *d_count, float *h_mean, float *h_rms,
                        void *stream);
+
+/* Observation parameters of an <observation> XML file in either dialect
+ * (processSurveyParameters of src/mdsm/dedispersion_manager.cpp:40-200 and
+ * src/mdsm-multibeam/multibeam_dedispersion_manager.cpp:69-321), first
+ * beam only, as plain C: what the single-beam (legacy) manager reads. */
+typedef struct {
+  float fch1, foff, tsamp, lowdm, dmstep, detection_threshold;
+  uint32_t nchans, tdms, nsamp, nbits;
+  int use_brute_force;            /* dm@useBruteForce; 1 when absent */
+  char fileprefix[120], basedir[120];
+  uint32_t secs_per_file;
+  char use_pc_time, single_file_mode;
+  int dialect;                    /* MDSM_XML_* of mdsm_compat.h: 1 multibeam, 2 legacy, 3 mixed */
+} mdsm_xml_params_t;
+int mdsm_parse_observation_xml(const char *path, mdsm_xml_params_t *out);
 
 /* 5-tap running median per row, edges replicated (median_filter,
  * multibeam_dedispersion_kernel.cu:98-157), out of place. */

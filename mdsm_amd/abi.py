@@ -103,7 +103,7 @@ _sig("mdsm_normalise", _vp, _u64, _u32, _u64, _vp, _vp, _vp, _i, _u64, _vp)
 _sig("mdsm_block_stats", _vp, _u32, _u32, _u32, _vp, _vp, _vp)
 _sig("mdsm_detect", _vp, _u32, _u32, _u32, _vp, _vp, _i, _d, _vp, _u32, _vp, _vp)
 _sig("mdsm_detect_global", _vp, _u32, _u32, _f, _vp, _u32, _vp, _P(_d), _P(_d), _vp)
-_sig("mdsm_detect_legacy", _vp, _u32, _u32, _f, _f, _f, _vp, _vp, _u32, _u32, _vp, _u32, _vp,
+_sig("mdsm_detect_legacy", _vp, _u32, _u32, _f, _f, _f, _u64, _vp, _vp, _u32, _u32, _vp, _u32, _vp,
      _P(_f), _P(_f), _vp)
 _sig("mdsm_median_filter5", _vp, _u32, _u32, _vp, _vp)
 _sig("mdsm_detrend_normalise", _vp, _u64, _u32, _vp)
@@ -324,11 +324,11 @@ def detect_global(d_x, nrows, nsamp, thresh, d_cands, cap, d_count, stream=None)
 
 
 def detect_legacy(d_x, ndms, nsamp, lowdm, dmstep, thresh, d_cands, cap, d_count, d_mean=None,
-                  d_rms=None, samples_per_chunk=0, nstats=0, stream=None):
+                  d_rms=None, samples_per_chunk=0, nstats=0, stream=None, nvals=0):
     """Legacy median-of-five detector; returns the (mean, rms) it used, or
     None with caller-supplied per-chunk statistics."""
     m, r = _f(0.0), _f(0.0)
-    check(_lib.mdsm_detect_legacy(d_x, ndms, nsamp, lowdm, dmstep, thresh, d_mean, d_rms,
+    check(_lib.mdsm_detect_legacy(d_x, ndms, nsamp, lowdm, dmstep, thresh, nvals, d_mean, d_rms,
                                   samples_per_chunk, nstats, d_cands, cap, d_count, C.byref(m),
                                   C.byref(r), stream))
     return None if d_mean is not None else (m.value, r.value)

@@ -507,7 +507,7 @@ extern "C" int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsa
 }
 
 extern "C" int mdsm_detect_legacy(const float *d_x, uint32_t ndms, uint32_t nsamp, float lowdm,
-                                  float dmstep, float thresh, const float *d_mean,
+                                  float dmstep, float thresh, uint64_t nvals, const float *d_mean,
                                   const float *d_rms, uint32_t samples_per_chunk, uint32_t nstats,
                                   mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count,
                                   float *h_mean, float *h_rms, void *stream) {
@@ -529,7 +529,8 @@ extern "C" int mdsm_detect_legacy(const float *d_x, uint32_t ndms, uint32_t nsam
     // dedispersion_output.cpp:145-166: blocks of mod_factor values; the loop
     // leaves after block n once (n + 1) * mod_factor >= vals, so with more
     // than one block the last block (full or partial) never enters the mean
-    const uint32_t vals = ndms * nsamp;
+    MDSM_REQUIRE(nvals <= (uint64_t)ndms * nsamp, MDSM_ERR_ARG, "nvals exceeds ndms * nsamp");
+    const uint32_t vals = nvals ? (uint32_t)nvals : ndms * nsamp;
     const uint32_t mod = vals < 32u * 1024u ? vals : 32u * 1024u;
     const uint32_t nfull = (uint32_t)(((uint64_t)vals + mod - 1) / mod);
     const uint32_t nblocks = nfull > 1 ? nfull - 1 : 1;

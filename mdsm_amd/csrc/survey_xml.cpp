@@ -118,6 +118,7 @@ using mdsm_xml::Element;
 
 static thread_local std::string g_xml_err;
 static int g_dialect = 0;
+static int g_brute_force = 1;  // dm@useBruteForce of the last file read (legacy dialect)
 
 extern "C" const char *mdsm_compat_xml_error(void) { return g_xml_err.c_str(); }
 extern "C" int mdsm_compat_xml_dialect(void) { return g_dialect; }
@@ -205,6 +206,7 @@ SURVEY *processSurveyParameters(const std::string &filepath) {
   s->output_compression = 1;
   s->npols = 1;
   int dialect = 0;
+  g_brute_force = 1;
   std::vector<BEAM> beams;
   float top = 0, offset = 0;
   bool have_freq = false;
@@ -213,6 +215,9 @@ SURVEY *processSurveyParameters(const std::string &filepath) {
     Attrs a{e, &dialect};
     const std::string &n = e->name;
     if (n == "dm") {
+      auto bf = e->attrs.find("usebruteforce");
+      if (bf != e->attrs.end() && !bf->second.empty())
+        g_brute_force = std::strtol(bf->second.c_str(), nullptr, 10) != 0;
       a.f(s->lowdm, "lowDM");
       a.u(s->tdms, "numDMs");
       a.f(s->dmstep, "dmStep");
@@ -341,6 +346,37 @@ extern "C" int mdsm_compat_parse_xml(const char *path, mdsm_survey_t *beams, uin
   if (threshold) *threshold = s->detection_threshold;
   if (min_points) *min_points = s->dbscan_min_points;
   if (dialect) *dialect = g_dialect;
+  free(s->beams);
+  free(s->gpu_ids);
+  free(s->channel_mask);
+  free(s);
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_parse_observation_xml(const char *path, mdsm_xml_params_t *out) {
+  if (!out) return MDSM_ERR_ARG;
+  SURVEY *s = processSurveyParameters(path ? std::string(path) : std::string());
+  if (!s) return MDSM_ERR_ARG;
+  memset(out, 0, sizeof(*out));
+  if (s->nbeams) {
+    out->fch1 = s->beams[0].fch1;
+    out->foff = s->beams[0].foff;
+  }
+  out->tsamp = s->tsamp;
+  out->lowdm = s->lowdm;
+  out->dmstep = s->dmstep;
+  out->detection_threshold = s->detection_threshold;
+  out->nchans = s->nchans;
+  out->tdms = s->tdms;
+  out->nsamp = s->nsamp;
+  out->nbits = s->nbits;
+  out->use_brute_force = g_brute_force;
+  strncpy(out->fileprefix, s->fileprefix, sizeof(out->fileprefix) - 1);
+  strncpy(out->basedir, s->basedir, sizeof(out->basedir) - 1);
+  out->secs_per_file = s->secs_per_file;
+  out->use_pc_time = s->use_pc_time;
+  out->single_file_mode = s->single_file_mode;
+  out->dialect = g_dialect;
   free(s->beams);
   free(s->gpu_ids);
   free(s->channel_mask);

@@ -84,7 +84,7 @@ _sig("oracle_dbscan2d", _i, _i32p, _i32p, _i, _i, _i, _i32p)
 _sig("oracle_dbscan_ref", _i, _f64p, _f64p, _f64p, _u32, _f, _f, _f, _u32, _i32p)
 _sig("oracle_median5", _f, _f, _f, _f, _f, _f)
 _sig("oracle_median_filter5", None, _f32p, _i, _i, _f32p)
-_sig("oracle_process_brute_legacy", _i64, _f32p, _i, _i, _f, _f, _f, _i64, _i32p, _i32p,
+_sig("oracle_process_brute_legacy", _i64, _f32p, _i, _i, _f, _f, _f, _i64, _i64, _i32p, _i32p,
      _f32p, C.POINTER(_f), C.POINTER(_f))
 _sig("oracle_process_brute_legacy_chunks", _i64, _f32p, _i, _i, _f, _f, _f, _f32p, _f32p, _i,
      _i64, _i32p, _i32p, _f32p)
@@ -341,7 +341,7 @@ def median_filter5(x):
     return out
 
 
-def process_brute_legacy(x, lowdm, dmstep, thresh, cap=1 << 20):
+def process_brute_legacy(x, lowdm, dmstep, thresh, cap=1 << 20, nvals=0):
     x = _c(x, np.float32)
     ndms, nsamp = x.shape
     cd = np.empty(cap, np.int32)
@@ -349,7 +349,7 @@ def process_brute_legacy(x, lowdm, dmstep, thresh, cap=1 << 20):
     cs = np.empty(cap, np.float32)
     m = C.c_float()
     r = C.c_float()
-    n = _lib.oracle_process_brute_legacy(x, ndms, nsamp, lowdm, dmstep, thresh, cap, cd, ct, cs,
+    n = _lib.oracle_process_brute_legacy(x, ndms, nsamp, lowdm, dmstep, thresh, nvals, cap, cd, ct, cs,
                                          C.byref(m), C.byref(r))
     k = min(n, cap)
     return cd[:k].copy(), ct[:k].copy(), cs[:k].copy(), int(n), m.value, r.value

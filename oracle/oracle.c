@@ -754,9 +754,11 @@ void oracle_median_filter5(const float *x, int nrows, int nsamp, float *out) {
 /* Legacy detector (dedispersion_output.cpp:135-213, mean == NULL branch):
  * mean/rms from 32K-value blocks, median of five, threshold with thisdm > 1. */
 int64_t oracle_process_brute_legacy(const float *x, int ndms, int nsamp, float lowdm, float dmstep,
-                                    float thresh, int64_t cap, int32_t *cd, int32_t *ct, float *csnr,
-                                    float *out_mean, float *out_rms) {
-  unsigned vals = (unsigned)nsamp * (unsigned)ndms;
+                                    float thresh, int64_t nvals, int64_t cap, int32_t *cd,
+                                    int32_t *ct, float *csnr, float *out_mean, float *out_rms) {
+  /* vals = read_nsamp * tdms (:147): the first nvals values of the row-major
+   * array enter the statistics (all of them when nvals is 0) */
+  unsigned vals = nvals ? (unsigned)nvals : (unsigned)nsamp * (unsigned)ndms;
   unsigned mod_factor = vals < 32 * 1024 ? vals : 32 * 1024;
   unsigned iters = 0, j;
   double mean2 = 0, total, total2;

@@ -140,7 +140,7 @@ def test_detect_global_reference_semantics():
     assert list(zip(c["dm_idx"], c["sample"])) == list(zip(od, ot))
 
 
-def _legacy_gpu(x, lowdm, dmstep, thresh, mean=None, rms=None, spc=0, offset=0):
+def _legacy_gpu(x, lowdm, dmstep, thresh, mean=None, rms=None, spc=0, offset=0, nvals=0):
     ndms, nsamp = x.shape
     # offset shifts the device copy by whole floats (16-byte misalignment)
     d_x = abi.DeviceBuffer.from_numpy(np.concatenate([np.zeros(offset, np.float32), x.ravel()]))
@@ -150,11 +150,11 @@ def _legacy_gpu(x, lowdm, dmstep, thresh, mean=None, rms=None, spc=0, offset=0):
     d_r = abi.DeviceBuffer.from_numpy(rms) if rms is not None else None
     st = abi.detect_legacy(d_x.ptr + 4 * offset, ndms, nsamp, lowdm, dmstep, thresh, d_c.ptr, cap,
                            d_n.ptr, d_m.ptr if d_m else None, d_r.ptr if d_r else None, spc,
-                           0 if mean is None else mean.size)
+                           0 if mean is None else mean.size, nvals=nvals)
     return st, abi.read_candidates(d_c, d_n, cap)
 
 
-@pytest.mark.parametrize("case", ["pulse", "odd_small", "quirk", "misaligned", "short_rows"])
+@pytest.mark.parametrize("case", ["pulse", "odd_small", "quirk", "misaligned", "short_rows", "prefix"])
 def test_detect_legacy_bitexact(case):
     """Global-statistics branch of the legacy detector: mean and rms equal
     the oracle's FP32 values exactly, candidates (row, sample, snr) too."""
@@ -174,12 +174,19 @@ def test_detect_legacy_bitexact(case):
         x = (rng.standard_normal((9, 32771)) * 3 + 1).astype(np.float32)
         args = (0.5, 0.25, 2.5)
         off = 1
-    else:
+    elif case == "short_rows":
         x = rng.standard_normal((4, 5)).astype(np.float32) * 4
         args = (1.5, 1.0, 0.0)
-    _say("legacy", case, x.shape)
-    (m, r), c = _legacy_gpu(x, *args, offset=off)
-    od, ot, osnr, on, om, orms = O.process_brute_legacy(x, *args)
+    else:
+        # statistics over a prefix of the array (a short last chunk:
+        # read_nsamp * ndms values), threshold over every row
+        x = (rng.standard_normal((5, 40000)) + np.arange(5)[:, None]).astype(np.float32)
+        args = (0.0, 0.5, 2.0)
+        nv = 3 * 40000 + 777
+    nv = nv if case == "prefix" else 0
+    _say("legacy", case, x.shape, nv)
+    (m, r), c = _legacy_gpu(x, *args, offset=off, nvals=nv)
+    od, ot, osnr, on, om, orms = O.process_brute_legacy(x, *args, nvals=nv)
     assert (m, r) == (om, orms), (m, r, om, orms)
     assert len(c) == on
     assert c["dm_idx"].tolist() == od.tolist() and c["sample"].tolist() == ot.tolist()
