@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 from mdsm_amd import abi  # noqa: E402
 
 HBM_PEAK = 8.0e12        # MI355X HBM3E, bytes/s
+LDS_PEAK = 256 * 128 * 2.4e9  # bytes/s: 128 B per clock per CU, 256 CUs at 2.4 GHz
 VALU_ADD_PEAK = 39.3e12  # FP32 vector adds/s, one per lane per clock (157.3 TFLOP/s counts
                          # packed FMA as 4 flops; packed FP32 is not used here)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch_fp16.json")
@@ -688,6 +689,10 @@ def main():
                      "algorithmic_bytes_per_launch": B16, "measured_copy_gbs": copy_gbs},
         "valu": {"adds_per_s": total_adds / (ms16 / 1e3), "peak_adds_per_s": VALU_ADD_PEAK,
                  "frac": total_adds / (ms16 / 1e3) / VALU_ADD_PEAK},
+        # algorithmic LDS reads of the FP16 tiles: half a 4-byte word (one
+        # 16-bit sample) per add
+        "lds": {"achieved": 2.0 * total_adds / (ms16 / 1e3), "peak": LDS_PEAK, "unit": "B/s",
+                "frac": 2.0 * total_adds / (ms16 / 1e3) / LDS_PEAK, "bytes_per_add": 2},
         "batched_u8": u8,
         "batched_u8_vs_plain_fp32": (u8["dm_samples_per_s"] / plain["dm_samples_per_s"]) if (u8 and plain) else None,
         "plain_fp32_per_beam": plain,
