@@ -67,3 +67,18 @@ def test_observation_xml_subband_flag(tmp_path):
     assert rc == 0 and x.use_brute_force == 1
     rc, _ = _parse(str(tmp_path / "missing.xml"))
     assert rc != 0
+
+
+def test_legacy_open_refuses_subband(tmp_path):
+    # refused before any device call: subband dedispersion is not provided
+    p = tmp_path / "sub.xml"
+    p.write_text('<observation><frequencies top="400" offset="-1"/>'
+                 '<dm lowDM="0" numDMs="8" dmStep="1" useBruteForce="0"/><channels number="8"/>'
+                 '<timing tsamp="0.001"/><samples number="1024"/></observation>')
+    lib = C.CDLL(LEGACY)
+    lib.mdsm_legacy_open.argtypes = [C.c_char_p, C.c_uint, C.c_uint]
+    lib.mdsm_legacy_last_error.restype = C.c_char_p
+    assert lib.mdsm_legacy_open(str(p).encode(), 0, 0) != 0
+    assert b"subband" in lib.mdsm_legacy_last_error()
+    assert lib.mdsm_legacy_open(str(tmp_path / "missing.xml").encode(), 0, 0) != 0
+    lib.mdsm_legacy_close()
