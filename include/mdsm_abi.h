@@ -128,7 +128,9 @@ int mdsm_plan_device_table(mdsm_plan plan, const int32_t **table);
 int mdsm_plan_set_variant(mdsm_plan plan, int variant);
 int mdsm_plan_get_variant(mdsm_plan plan, int *variant);
 int mdsm_tile_variants(int in_dtype, int *count);
-/* Time each tile variant on the given buffers and keep the fastest. */
+/* Time each tile variant on the given buffers (best of three launches after
+ * a warm-up) and keep the fastest; the storage type's default variant is
+ * kept when it is within 2 % of the fastest. */
 int mdsm_plan_autotune(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out, void *stream,
                        int *best_variant, float *best_ms);
 
@@ -284,6 +286,14 @@ int mdsm_batch_launch(mdsm_batch batch, void *stream);
 int mdsm_batch_dedisperse(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                           float *const *d_out, void *stream);
 int mdsm_batch_set_variant(mdsm_batch batch, int variant);
+/* Tile autotuner for a batch (the reference has none: its tile is fixed at
+ * 128 x 32, multibeam_dedispersion_thread.cu:204-221): times every tile shape
+ * on the beam with the most channel x DM work, truncated to 2^17 samples,
+ * reading that beam's input d_in[big] (ld[big]) into a scratch output it
+ * allocates, and keeps the fastest shape that fits every beam (the next bind
+ * uses it).  best_variant receives the batch's variant afterwards. */
+int mdsm_batch_autotune(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
+                        void *stream, int *best_variant, float *best_ms);
 /* Launch shape (takes effect at the next bind): FLAT (default) launches one
  * workgroup per tile, numbered so that the DM tiles re-reading one input
  * window share an XCD (and its L2); PERSISTENT runs resident workgroups that

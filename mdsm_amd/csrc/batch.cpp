@@ -148,6 +148,48 @@ extern "C" int mdsm_batch_set_variant(mdsm_batch batch, int variant) {
   return MDSM_OK;
 }
 
+// Tile autotuner for a batch: the beam with the most DM x channel work,
+// truncated to 2^17 samples, is dedispersed with every tile shape
+// (mdsm_plan_autotune on the caller's input of that beam, into a scratch
+// output allocated here), and the fastest shape that fits every beam is kept.
+extern "C" int mdsm_batch_autotune(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
+                                   void *stream, int *best_variant, float *best_ms) {
+  MDSM_REQUIRE(batch && d_in && ld, MDSM_ERR_ARG, "null argument");
+  const uint32_t nb = (uint32_t)batch->beams.size();
+  uint32_t big = 0;
+  for (uint32_t i = 1; i < nb; ++i)
+    if ((uint64_t)batch->beams[i].nchans * batch->beams[i].ndms >
+        (uint64_t)batch->beams[big].nchans * batch->beams[big].ndms)
+      big = i;
+  MDSM_REQUIRE(d_in[big], MDSM_ERR_ARG, "null input for beam %u", big);
+  mdsm_survey_t probe = batch->beams[big];
+  probe.nsamp = std::min<uint32_t>(probe.nsamp, 1u << 17);
+  mdsm_plan p = nullptr;
+  int rc = mdsm_plan_create(&probe, batch->dtype, &p);
+  if (rc != MDSM_OK) return rc;
+  float *d_out = nullptr;
+  if (hipMalloc((void **)&d_out, sizeof(float) * probe.ndms * probe.nsamp) != hipSuccess) {
+    mdsm_plan_destroy(p);
+    set_error("autotune: scratch output allocation failed");
+    return MDSM_ERR_NOMEM;
+  }
+  int v = 0;
+  float ms = 0.f;
+  rc = mdsm_plan_autotune(p, d_in[big], ld[big], d_out, stream, &v, &ms);
+  (void)hipFree(d_out);
+  mdsm_plan_destroy(p);
+  if (rc != MDSM_OK) return rc;
+  bool fits = v > 0;
+  for (uint32_t i = 0; i < nb && fits; ++i) fits = batch->row_words[i][v] != 0;
+  if (fits) {
+    batch->variant = v;
+    batch->bound = false;
+  }
+  if (best_variant) *best_variant = batch_variant(batch);
+  if (best_ms) *best_ms = ms;
+  return MDSM_OK;
+}
+
 extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                                float *const *d_out) {
   MDSM_REQUIRE(batch && d_in && ld && d_out, MDSM_ERR_ARG, "null argument");

@@ -54,11 +54,15 @@ def test_batch_uneven_beams_bitexact(dtype):
         outs.append(do)
     for mode in (abi.BATCH_PERSISTENT, abi.BATCH_FLAT):
         batch.set_mode(mode)
-        for v in (0, 1, 4, 9, 12, 13):
+        for v in (0, 1, 4, 9, 12, 13, 15, "tuned"):
             for b, s in enumerate(beams):  # every tile must be written again
                 outs[b].upload(np.full(s.ndms * s.nsamp, np.nan, np.float32))
             print(f"[batch dtype {dtype}] mode {mode} variant {v}", flush=True, file=sys.stderr)
-            batch.set_variant(v)
+            if v == "tuned":  # the library's batch autotuner picks and keeps a shape
+                got_v, ms = batch.autotune([p for p, _ in ins], [l for _, l in ins])
+                assert 1 <= got_v <= abi.tile_variants() and ms > 0
+            else:
+                batch.set_variant(v)
             batch.dedisperse([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
             abi.synchronize()
             for b, s in enumerate(beams):
