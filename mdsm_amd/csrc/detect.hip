@@ -16,13 +16,38 @@ namespace {
 
 constexpr int NT = 256;
 
-__device__ __forceinline__ void tree256(double *p1, double *p2) {
+// Sum of the block's 256 per-thread partials (a, q) in one fixed pairing,
+// p[i] += p[i + s] for s = 128, 64, ..., 1: the two levels that cross waves
+// go through LDS, the six inside the first wave are wavefront shuffles (lane
+// l adds lane l + s), so the order of the FP64 adds is the oracle's.  The
+// totals land in p1[0] and p2[0], visible to every thread on return.
+__device__ __forceinline__ void tree256(double a, double q, double *p1, double *p2) {
   const int i = threadIdx.x;
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    __syncthreads();
-    if (i < s) {
-      p1[i] = p1[i] + p1[i + s];
-      p2[i] = p2[i] + p2[i + s];
+  if (i >= 128) {
+    p1[i] = a;
+    p2[i] = q;
+  }
+  __syncthreads();
+  if (i < 128) {
+    a = a + p1[i + 128];
+    q = q + p2[i + 128];
+    if (i >= 64) {
+      p1[i] = a;
+      p2[i] = q;
+    }
+  }
+  __syncthreads();
+  if (i < 64) {
+    a = a + p1[i + 64];
+    q = q + p2[i + 64];
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+      a = a + __shfl_down(a, s);
+      q = q + __shfl_down(q, s);
+    }
+    if (i == 0) {
+      p1[0] = a;
+      p2[0] = q;
     }
   }
   __syncthreads();
@@ -50,9 +75,7 @@ __global__ __launch_bounds__(NT) void channel_stats_kernel(const float *__restri
     a = a + v;
     b = b + v * v;
   }
-  p1[threadIdx.x] = a;
-  p2[threadIdx.x] = b;
-  tree256(p1, p2);
+  tree256(a, b, p1, p2);
   if (threadIdx.x == 0) moments(p1[0], p2[0], n, &mean[c], &stdv[c]);
 }
 
@@ -98,9 +121,7 @@ __global__ __launch_bounds__(NT) void block_stats_kernel(const float *__restrict
     a = a + v;
     q = q + v * v;
   }
-  p1[threadIdx.x] = a;
-  p2[threadIdx.x] = q;
-  tree256(p1, p2);
+  tree256(a, q, p1, p2);
   if (threadIdx.x == 0) moments(p1[0], p2[0], n, &mean[(uint64_t)d * nb + b], &stdv[(uint64_t)d * nb + b]);
 }
 
@@ -201,9 +222,7 @@ __global__ __launch_bounds__(NT) void global_sums_kernel(const float *__restrict
     a = a + v;
     b = b + v * v;
   }
-  p1[threadIdx.x] = a;
-  p2[threadIdx.x] = b;
-  tree256(p1, p2);
+  tree256(a, b, p1, p2);
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = p1[0];
     partials[2 * blockIdx.x + 1] = p2[0];
@@ -355,9 +374,7 @@ __global__ __launch_bounds__(NT) void detrend_kernel(float *__restrict__ x, uint
     sy = sy + (double)y[i];
     sxy = sxy + xi * (double)y[i];
   }
-  p1[threadIdx.x] = sy;
-  p2[threadIdx.x] = sxy;
-  tree256(p1, p2);
+  tree256(sy, sxy, p1, p2);
   // sum of x_i^2 for x_i = i - (L-1)/2 is (L^3 - L)/12
   const double sxx = ((double)L * (double)L * (double)L - (double)L) / 12.0;
   const double a = p1[0] / (double)L, b = p2[0] / sxx;
@@ -367,9 +384,7 @@ __global__ __launch_bounds__(NT) void detrend_kernel(float *__restrict__ x, uint
     const double r = (double)y[i] - (a + b * (half + (double)i));
     ss = ss + r * r;
   }
-  p1[threadIdx.x] = ss;
-  p2[threadIdx.x] = 0.0;
-  tree256(p1, p2);
+  tree256(ss, 0.0, p1, p2);
   const double sd = sqrt(p1[0] / (double)L);
   for (uint32_t i = threadIdx.x; i < L; i += NT) {
     const double r = (double)y[i] - (a + b * (half + (double)i));
