@@ -157,14 +157,40 @@ __global__ __launch_bounds__(NT) void detect_kernel(const float *__restrict__ x,
                                                     const double *__restrict__ stdv, int nwidths,
                                                     double thresh, mdsm_cand_t *__restrict__ cands,
                                                     uint32_t cap, uint32_t *__restrict__ count) {
-  __shared__ double lev[DT_T + DT_HALO];
+  constexpr int NE = DT_T + DT_HALO;           // staged samples (tile + halo)
+  constexpr int NQ = (NE + NT - 1) / NT;       // of them per thread, i = tid + q*NT
+  __shared__ double lev[NE];
   const uint32_t d = blockIdx.y;
   const uint32_t t0 = blockIdx.x * DT_T;
   const float *row = x + (uint64_t)d * nsamp;
-  for (uint32_t i = threadIdx.x; i < DT_T + DT_HALO; i += NT) {
+  // this thread's values of the current level, kept in registers (LDS holds
+  // them for the neighbours' t + 2^(k-1) reads)
+  double cur[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const uint32_t i = threadIdx.x + q * NT;
     const uint32_t t = t0 + i;
-    lev[i] = t < nsamp ? (double)row[t] : 0.0;
+    cur[q] = (i < NE && t < nsamp) ? (double)row[t] : 0.0;
+    if (i < NE) lev[i] = cur[q];
   }
+  // block statistics of the thread's own samples, loaded once
+  double m[DT_Q], sd[DT_Q];
+#pragma unroll
+  for (int q = 0; q < DT_Q; ++q) {
+    const uint32_t t = t0 + threadIdx.x + q * NT;
+    const uint64_t b = (uint64_t)d * nb + (t < nsamp ? t / L : 0u);
+    m[q] = mean[b];
+    sd[q] = stdv[b];
+  }
+  // A level can only be the maximum of a candidate if its S/N reaches thresh.
+  // a >= lim * den (lim = thresh * (1 - 2^-40), thresh > 0) never rejects
+  // such a level: correctly rounded a / den >= thresh needs
+  // a >= thresh * den * (1 - 2^-53), above the rounded bound; and a level it
+  // rejects has a / den < thresh * (1 - 2^-41), so its quotient rounds below
+  // thresh.  The exact FP64 division then runs only for the levels that pass,
+  // and best / bk equal those of dividing at every level.  Outside
+  // 2^-960 < den < 2^960 every level divides.
+  const double lim = thresh > 0.0 ? thresh * (1.0 - 0x1p-40) : -INFINITY;
   double best[DT_Q];
   int bk[DT_Q];
 #pragma unroll
@@ -172,32 +198,33 @@ __global__ __launch_bounds__(NT) void detect_kernel(const float *__restrict__ x,
   for (int k = 0; k < nwidths; ++k) {
     const uint32_t w = 1u << k;
     if (k > 0) {
+      // S_k(i) = S_{k-1}(i) + S_{k-1}(i + 2^(k-1))
       const uint32_t h = w >> 1;
-      double v[(DT_T + DT_HALO + NT - 1) / NT];
+      double nv[NQ];
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < (DT_T + DT_HALO + NT - 1) / NT; ++q) {
+      for (int q = 0; q < NQ; ++q) {
         const uint32_t i = threadIdx.x + q * NT;
-        v[q] = (i + h < DT_T + DT_HALO) ? lev[i] + lev[i + h] : 0.0;
+        nv[q] = (i + h < (uint32_t)NE) ? cur[q] + lev[i + h] : 0.0;
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < (DT_T + DT_HALO + NT - 1) / NT; ++q) {
+      for (int q = 0; q < NQ; ++q) {
         const uint32_t i = threadIdx.x + q * NT;
-        if (i < DT_T + DT_HALO) lev[i] = v[q];
+        cur[q] = nv[q];
+        if (i < (uint32_t)NE) lev[i] = cur[q];
       }
     }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < DT_Q; ++q) {
-      const uint32_t i = threadIdx.x + q * NT;
-      const uint32_t t = t0 + i;
-      if ((uint64_t)t + w <= nsamp) {
-        const uint32_t b = t / L;
-        const double m = mean[(uint64_t)d * nb + b];
-        const double s = stdv[(uint64_t)d * nb + b];
-        if (s > 0.0) {
-          const double snr = (lev[i] - (double)w * m) / (s * kSqrtW[k]);
+      const uint32_t t = t0 + threadIdx.x + q * NT;
+      if ((uint64_t)t + w <= nsamp && sd[q] > 0.0) {
+        const double a = cur[q] - (double)w * m[q];
+        const double den = sd[q] * kSqrtW[k];
+        // (the bound's relative error holds while lim * den stays normal)
+        const bool in_range = den > 0x1p-960 && den < 0x1p960;
+        if (!in_range || a >= lim * den) {
+          const double snr = a / den;
           if (snr > best[q]) { best[q] = snr; bk[q] = k; }
         }
       }

@@ -221,3 +221,24 @@ def test_detrend_vs_polyfit():
         b, a = np.polyfit(xi, seg, 1)
         r = seg - (a + b * xi)
         assert np.allclose(got[s * L:(s + 1) * L], r / np.sqrt((r * r).mean()), atol=1e-5)
+
+
+def test_detection_division_filter_never_rejects_a_passing_level():
+    # The boxcar detector skips the FP64 division of a level unless
+    # a >= thresh * (1 - 2^-40) * den; that must hold whenever the correctly
+    # rounded a / den reaches thresh, and a rejected level must round below
+    # thresh (numpy float64 is IEEE double with round-to-nearest-even).
+    rng = np.random.default_rng(11)
+    n = 400000
+    thresh = rng.uniform(0.5, 20.0, n)
+    den = np.exp(rng.uniform(np.log(1e-6), np.log(1e6), n))
+    # quotients placed within a few ulps of the threshold, and spread widely
+    near = thresh * den * (1.0 + rng.integers(-8, 9, n) * 2.0 ** -52)
+    far = thresh * den * rng.uniform(0.5, 1.5, n)
+    for a in (near, far):
+        q = a / den
+        lim = thresh * (1.0 - 2.0 ** -40)
+        passes = a >= lim * den
+        assert not np.any((q >= thresh) & ~passes)
+        assert np.all(q[~passes] < thresh[~passes])
+        assert (~passes).any() or a is near
