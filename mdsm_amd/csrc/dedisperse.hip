@@ -345,8 +345,8 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
       }
     };
     // build the pair copies of rows [cb, cb+nc) in buffer b
-    // (F16: wave w builds rows w, w+4, ..., two Q words per lane from one
-    // 8-byte and one 4-byte LDS read; Q[wp-1], which no trial reads, may
+    // (F16: wave w builds rows w, w+4, ..., four Q words per lane from one
+    // 16-byte and one 4-byte LDS read; Q[wp-1], which no trial reads, may
     // take the stale word past P.  U8: also one wave per row.)
     auto build = [&](uint32_t cb, uint32_t *b) {
       const uint32_t nc = min(cbr, nchans - cb);
@@ -369,11 +369,14 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
         for (uint32_t r = wave; r < nc; r += 4u) {
           uint32_t *P = b + r * rw;
           uint32_t *Q = P + wp;
-          for (uint32_t w = 2u * lane; w + 1u < wp; w += 128u) {
-            const uint2 a = *reinterpret_cast<const uint2 *>(P + w);
-            const uint32_t c = P[w + 2];
-            *reinterpret_cast<uint2 *>(Q + w) = make_uint2(__builtin_amdgcn_alignbit(a.y, a.x, 16),
-                                                           __builtin_amdgcn_alignbit(c, a.y, 16));
+          // four Q words per lane (wp is a multiple of 64, rows 16-byte aligned)
+          for (uint32_t w = 4u * lane; w + 3u < wp; w += 256u) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(P + w);
+            const uint32_t c = P[w + 4];
+            *reinterpret_cast<uint4 *>(Q + w) = make_uint4(__builtin_amdgcn_alignbit(a.y, a.x, 16),
+                                                           __builtin_amdgcn_alignbit(a.z, a.y, 16),
+                                                           __builtin_amdgcn_alignbit(a.w, a.z, 16),
+                                                           __builtin_amdgcn_alignbit(c, a.w, 16));
           }
         }
       }
