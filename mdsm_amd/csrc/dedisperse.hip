@@ -351,18 +351,24 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
     auto build = [&](uint32_t cb, uint32_t *b) {
       const uint32_t nc = min(cbr, nchans - cb);
       if (IS_U8) {
-        // wave w widens rows w, w+4, ...: lane n turns raw dwords n, n+1
-        // into P words 2n, 2n+1 and Q words 2n, 2n+1 (two 8-byte writes)
+        // wave w widens rows w, w+4, ...: raw dwords n, n+1 (and n+2 for
+        // the last Q word) become P words 2n..2n+3 and Q words 2n..2n+3
+        // (two 16-byte writes)
         for (uint32_t r = wave; r < nc; r += 4u) {
           uint32_t *P = b + r * rw;
           uint32_t *Q = P + wp;
           const uint32_t *R = raw + r * rawn;
-          for (uint32_t n = lane; 2u * n < wp; n += 64u) {
-            const uint32_t x = R[n], y = R[n + 1];
-            *reinterpret_cast<uint2 *>(P + 2 * n) =
-                make_uint2(__builtin_amdgcn_perm(y, x, 0x0C010C00u), __builtin_amdgcn_perm(y, x, 0x0C030C02u));
-            *reinterpret_cast<uint2 *>(Q + 2 * n) =
-                make_uint2(__builtin_amdgcn_perm(y, x, 0x0C020C01u), __builtin_amdgcn_perm(y, x, 0x0C040C03u));
+          // two raw dwords per lane (wp is a multiple of 64, so the four
+          // P and Q words of a lane are whole 16-byte groups)
+          for (uint32_t n = 2u * lane; 2u * n < wp; n += 128u) {
+            const uint2 xy = *reinterpret_cast<const uint2 *>(R + n);
+            const uint32_t x = xy.x, y = xy.y, z = R[n + 2];
+            *reinterpret_cast<uint4 *>(P + 2 * n) =
+                make_uint4(__builtin_amdgcn_perm(y, x, 0x0C010C00u), __builtin_amdgcn_perm(y, x, 0x0C030C02u),
+                           __builtin_amdgcn_perm(z, y, 0x0C010C00u), __builtin_amdgcn_perm(z, y, 0x0C030C02u));
+            *reinterpret_cast<uint4 *>(Q + 2 * n) =
+                make_uint4(__builtin_amdgcn_perm(y, x, 0x0C020C01u), __builtin_amdgcn_perm(y, x, 0x0C040C03u),
+                           __builtin_amdgcn_perm(z, y, 0x0C020C01u), __builtin_amdgcn_perm(z, y, 0x0C040C03u));
           }
         }
       } else {
