@@ -388,6 +388,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-beam FP32 pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the batched FP16 launches")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="tile variant of the batched FP16 launch instead of the autotuner's "
+                         "(for profiling one tile shape)")
     ap.add_argument("--tile", default=None,
                     help="only time one tile shape ('<f32|f16|u8>:<variant>') at 1024 ch x 2^18 x 4096, "
                          "the workload of the per-tile counter profiles")
@@ -520,7 +523,7 @@ def main():
             capture_rows(dtype, ins)
         return batch, ins, ms, B, v
 
-    batch16, ins16, ms16, B16, v16 = run_batch(abi.F16, "fp16")
+    batch16, ins16, ms16, B16, v16 = run_batch(abi.F16, "fp16", v=args.variant)
     modes16 = batch16.mode_ms
     checks = {}
     t0tab, _ = abi.delay_table(surveys[0])
@@ -661,10 +664,11 @@ def main():
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
         # counters of another tile shape do not describe this launch
-        if tr.get("tile_variant") == v16:
-            traffic = tr.get("bytes_per_launch")
+        entry = tr.get("by_variant", {}).get(str(v16))
+        if entry:
+            traffic = entry.get("bytes_per_launch")
         else:
-            log(f"traffic file profiled tile variant {tr.get('tile_variant')}, this run chose {v16}: "
+            log(f"no traffic profile of tile variant {v16} (profiled: {sorted(tr.get('by_variant', {}))}): "
                 "roofline.traffic left null")
     line = {
         "metric": "dedispersed DM x samples per second (batched, FP16 storage)",
