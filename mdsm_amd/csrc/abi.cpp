@@ -403,52 +403,65 @@ extern "C" int mdsm_dedisperse(mdsm_plan plan, const void *d_in, uint64_t ld, fl
                            as_stream(stream));
 }
 
-extern "C" int mdsm_plan_autotune(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out,
-                                  void *stream, int *best_variant, float *best_ms) {
-  MDSM_REQUIRE(plan && d_in && d_out, MDSM_ERR_ARG, "null argument");
-  hipStream_t s = as_stream(stream);
+namespace mdsm {
+
+// Time every tile variant of a plan on the given buffers: one warm-up launch,
+// then the best of three timed launches; t[v] < 0 where a variant does not fit.
+int plan_time_variants(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out,
+                       hipStream_t s, std::vector<float> &t) {
   hipEvent_t e0, e1;
   MDSM_HIP(hipEventCreate(&e0));
   MDSM_HIP(hipEventCreate(&e1));
-  // each shape: one warm-up launch, then the best of three timed launches;
-  // the storage type's default shape is kept unless another is more than
-  // 2 % faster, so that near-ties do not flip between runs
-  int best = 0;
-  float bms = 0.0f;
   const int nv = num_variants(plan->dtype);
-  std::vector<float> t(nv + 1, -1.0f);
+  const int keep = plan->variant;
+  t.assign(nv + 1, -1.0f);
   for (int v = 1; v <= nv; ++v) {
     plan->variant = v;
-    if (mdsm_dedisperse(plan, d_in, ld, d_out, stream) != MDSM_OK) continue;  // warm-up / LDS fit
+    if (mdsm_dedisperse(plan, d_in, ld, d_out, s) != MDSM_OK) continue;  // warm-up / LDS fit
     float vms = -1.0f;
     for (int rep = 0; rep < 3; ++rep) {
       float ms = 0.0f;
       MDSM_HIP(hipEventRecord(e0, s));
-      int rc = mdsm_dedisperse(plan, d_in, ld, d_out, stream);
+      int rc = mdsm_dedisperse(plan, d_in, ld, d_out, s);
       MDSM_HIP(hipEventRecord(e1, s));
       MDSM_HIP(hipEventSynchronize(e1));
       if (rc != MDSM_OK) break;
       MDSM_HIP(hipEventElapsedTime(&ms, e0, e1));
       if (vms < 0.0f || ms < vms) vms = ms;
     }
-    if (vms < 0.0f) continue;
     t[v] = vms;
-    if (!best || vms < bms) {
-      best = v;
-      bms = vms;
-    }
   }
+  plan->variant = keep;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  MDSM_REQUIRE(best > 0, MDSM_ERR_ARG, "no tile variant fits this survey");
-  const int pref = preferred_variant(plan->dtype);
-  if (pref != best && t[pref] > 0.0f && t[pref] <= 1.02f * bms) {
+  return MDSM_OK;
+}
+
+// The fastest entry of t (index 1..), or the storage type's default when
+// that is within 2 % of it, so that near-ties do not flip between runs.
+int pick_variant(int dtype, const std::vector<float> &t) {
+  int best = 0;
+  for (int v = 1; v < (int)t.size(); ++v)
+    if (t[v] >= 0.0f && (!best || t[v] < t[best])) best = v;
+  const int pref = preferred_variant(dtype);
+  if (best && pref != best && pref < (int)t.size() && t[pref] >= 0.0f && t[pref] <= 1.02f * t[best])
     best = pref;
-    bms = t[pref];
-  }
+  return best;
+}
+
+}  // namespace mdsm
+
+extern "C" int mdsm_plan_autotune(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out,
+                                  void *stream, int *best_variant, float *best_ms) {
+  MDSM_REQUIRE(plan && d_in && d_out, MDSM_ERR_ARG, "null argument");
+  std::vector<float> t;
+  int rc = plan_time_variants(plan, d_in, ld, d_out, as_stream(stream), t);
+  if (rc != MDSM_OK) return rc;
+  const int best = pick_variant(plan->dtype, t);
+  MDSM_REQUIRE(best > 0, MDSM_ERR_ARG, "no tile variant fits this survey");
   plan->variant = best;
   if (best_variant) *best_variant = best;
-  if (best_ms) *best_ms = bms;
+  if (best_ms) *best_ms = t[best];
   return MDSM_OK;
 }
 

@@ -148,45 +148,60 @@ extern "C" int mdsm_batch_set_variant(mdsm_batch batch, int variant) {
   return MDSM_OK;
 }
 
-// Tile autotuner for a batch: the beam with the most DM x channel work,
-// truncated to 2^17 samples, is dedispersed with every tile shape
-// (mdsm_plan_autotune on the caller's input of that beam, into a scratch
-// output allocated here), and the fastest shape that fits every beam is kept.
+// Tile autotuner for a batch: every beam, truncated to 2^15 samples, is
+// dedispersed with every tile shape (warm-up, best of three) from the
+// caller's input into a scratch output allocated here; a shape's batch time
+// is estimated as the sum of its beams' times scaled to their full length,
+// and the least estimate among the shapes that fit every beam is kept (the
+// default within 2 % of it).
 extern "C" int mdsm_batch_autotune(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                                    void *stream, int *best_variant, float *best_ms) {
   MDSM_REQUIRE(batch && d_in && ld, MDSM_ERR_ARG, "null argument");
   const uint32_t nb = (uint32_t)batch->beams.size();
-  uint32_t big = 0;
-  for (uint32_t i = 1; i < nb; ++i)
-    if ((uint64_t)batch->beams[i].nchans * batch->beams[i].ndms >
-        (uint64_t)batch->beams[big].nchans * batch->beams[big].ndms)
-      big = i;
-  MDSM_REQUIRE(d_in[big], MDSM_ERR_ARG, "null input for beam %u", big);
-  mdsm_survey_t probe = batch->beams[big];
-  probe.nsamp = std::min<uint32_t>(probe.nsamp, 1u << 17);
-  mdsm_plan p = nullptr;
-  int rc = mdsm_plan_create(&probe, batch->dtype, &p);
-  if (rc != MDSM_OK) return rc;
+  const int nv = num_variants(batch->dtype);
+  const uint32_t probe_n = 1u << 15;
+  size_t out_words = 0;
+  for (uint32_t i = 0; i < nb; ++i) {
+    MDSM_REQUIRE(d_in[i], MDSM_ERR_ARG, "null input for beam %u", i);
+    out_words = std::max(out_words, (size_t)batch->beams[i].ndms *
+                                        std::min<uint32_t>(batch->beams[i].nsamp, probe_n));
+  }
   float *d_out = nullptr;
-  if (hipMalloc((void **)&d_out, sizeof(float) * probe.ndms * probe.nsamp) != hipSuccess) {
-    mdsm_plan_destroy(p);
+  if (hipMalloc((void **)&d_out, sizeof(float) * out_words) != hipSuccess) {
     set_error("autotune: scratch output allocation failed");
     return MDSM_ERR_NOMEM;
   }
-  int v = 0;
-  float ms = 0.f;
-  rc = mdsm_plan_autotune(p, d_in[big], ld[big], d_out, stream, &v, &ms);
+  std::vector<double> est(nv + 1, 0.0);
+  std::vector<bool> ok(nv + 1, true);
+  ok[0] = false;
+  int rc = MDSM_OK;
+  for (uint32_t i = 0; i < nb && rc == MDSM_OK; ++i) {
+    mdsm_survey_t probe = batch->beams[i];
+    probe.nsamp = std::min<uint32_t>(probe.nsamp, probe_n);
+    mdsm_plan p = nullptr;
+    rc = mdsm_plan_create(&probe, batch->dtype, &p);
+    if (rc != MDSM_OK) break;
+    std::vector<float> t;
+    rc = plan_time_variants(p, d_in[i], ld[i], d_out, as_stream(stream), t);
+    mdsm_plan_destroy(p);
+    const double scale = (double)batch->beams[i].nsamp / (double)probe.nsamp;
+    for (int v = 1; v <= nv; ++v) {
+      if (t.size() <= (size_t)v || t[v] < 0.0f || !batch->row_words[i][v]) ok[v] = false;
+      else est[v] += scale * t[v];
+    }
+  }
   (void)hipFree(d_out);
-  mdsm_plan_destroy(p);
   if (rc != MDSM_OK) return rc;
-  bool fits = v > 0;
-  for (uint32_t i = 0; i < nb && fits; ++i) fits = batch->row_words[i][v] != 0;
-  if (fits) {
+  std::vector<float> tf(nv + 1, -1.0f);
+  for (int v = 1; v <= nv; ++v)
+    if (ok[v]) tf[v] = (float)est[v];
+  const int v = pick_variant(batch->dtype, tf);
+  if (v > 0) {
     batch->variant = v;
     batch->bound = false;
   }
   if (best_variant) *best_variant = batch_variant(batch);
-  if (best_ms) *best_ms = ms;
+  if (best_ms) *best_ms = v > 0 ? tf[v] : 0.0f;
   return MDSM_OK;
 }
 

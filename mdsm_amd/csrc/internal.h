@@ -114,6 +114,9 @@ int launch_threshold_global(const float *x, uint32_t nrows, uint32_t nsamp, doub
                             double rms, float thresh, mdsm_cand_t *cands, uint32_t cap,
                             uint32_t *count, hipStream_t s);
 int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t s);
+int plan_time_variants(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out,
+                       hipStream_t s, std::vector<float> &t);
+int pick_variant(int dtype, const std::vector<float> &t);
 int launch_median5(const float *x, uint32_t nrows, uint32_t nsamp, float *out, hipStream_t s);
 int launch_legacy_block_sums(const float *x, uint32_t block_len, uint32_t nblocks, double *sums,
                              hipStream_t s);
