@@ -288,11 +288,10 @@ int mdsm_batch_dedisperse(mdsm_batch batch, const void *const *d_in, const uint6
 int mdsm_batch_set_variant(mdsm_batch batch, int variant);
 /* Tile autotuner for a batch (the reference has none: its tile is fixed at
  * 128 x 32, multibeam_dedispersion_thread.cu:204-221): times every tile shape
- * on every beam truncated to 2^15 samples (reading d_in[i], ld[i] into a
+ * on every beam truncated to 2^16 samples (reading d_in[i], ld[i] into a
  * scratch output it allocates), estimates each shape's batch time as the
  * sum of its beams' times scaled to their full length, and keeps the least
- * estimate among the shapes that fit every beam, or the default within 2 %
- * of it (the next bind uses it).  best_variant receives the batch's variant
+ * estimate among the shapes that fit every beam (the next bind uses it).  best_variant receives the batch's variant
  * afterwards, best_ms the estimate. */
 int mdsm_batch_autotune(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                         void *stream, int *best_variant, float *best_ms);

@@ -148,18 +148,17 @@ extern "C" int mdsm_batch_set_variant(mdsm_batch batch, int variant) {
   return MDSM_OK;
 }
 
-// Tile autotuner for a batch: every beam, truncated to 2^15 samples, is
+// Tile autotuner for a batch: every beam, truncated to 2^16 samples, is
 // dedispersed with every tile shape (warm-up, best of three) from the
 // caller's input into a scratch output allocated here; a shape's batch time
 // is estimated as the sum of its beams' times scaled to their full length,
-// and the least estimate among the shapes that fit every beam is kept (the
-// default within 2 % of it).
+// and the least estimate among the shapes that fit every beam is kept.
 extern "C" int mdsm_batch_autotune(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
                                    void *stream, int *best_variant, float *best_ms) {
   MDSM_REQUIRE(batch && d_in && ld, MDSM_ERR_ARG, "null argument");
   const uint32_t nb = (uint32_t)batch->beams.size();
   const int nv = num_variants(batch->dtype);
-  const uint32_t probe_n = 1u << 15;
+  const uint32_t probe_n = 1u << 16;
   size_t out_words = 0;
   for (uint32_t i = 0; i < nb; ++i) {
     MDSM_REQUIRE(d_in[i], MDSM_ERR_ARG, "null input for beam %u", i);
@@ -192,10 +191,15 @@ extern "C" int mdsm_batch_autotune(mdsm_batch batch, const void *const *d_in, co
   }
   (void)hipFree(d_out);
   if (rc != MDSM_OK) return rc;
+  // the least estimate (a sum over every beam, steadier than one timing, so
+  // no preference for the default is needed)
   std::vector<float> tf(nv + 1, -1.0f);
-  for (int v = 1; v <= nv; ++v)
-    if (ok[v]) tf[v] = (float)est[v];
-  const int v = pick_variant(batch->dtype, tf);
+  int v = 0;
+  for (int u = 1; u <= nv; ++u)
+    if (ok[u]) {
+      tf[u] = (float)est[u];
+      if (!v || tf[u] < tf[v]) v = u;
+    }
   if (v > 0) {
     batch->variant = v;
     batch->bound = false;
