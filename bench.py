@@ -501,7 +501,8 @@ def main():
             st.free()
             abi.synchronize()
         if v is None:  # the library's batch autotuner
-            v, _ = batch.autotune([d.ptr for d, _ in ins], [l for _, l in ins], stream)
+            v, _ = batch.autotune([d.ptr for d, _ in ins], [l for _, l in ins], [o.ptr for o in outs],
+                                  stream)
         batch.set_variant(v)
         # launch shape: one timed launch of each, keep the faster
         mode_ms = {}

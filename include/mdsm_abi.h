@@ -287,14 +287,16 @@ int mdsm_batch_dedisperse(mdsm_batch batch, const void *const *d_in, const uint6
                           float *const *d_out, void *stream);
 int mdsm_batch_set_variant(mdsm_batch batch, int variant);
 /* Tile autotuner for a batch (the reference has none: its tile is fixed at
- * 128 x 32, multibeam_dedispersion_thread.cu:204-221): times every tile shape
- * on every beam truncated to 2^16 samples (reading d_in[i], ld[i] into a
- * scratch output it allocates), estimates each shape's batch time as the
- * sum of its beams' times scaled to their full length, and keeps the least
- * estimate among the shapes that fit every beam (the next bind uses it).  best_variant receives the batch's variant
- * afterwards, best_ms the estimate. */
+ * 128 x 32, multibeam_dedispersion_thread.cu:204-221), in two stages: every
+ * tile shape on every beam truncated to 2^16 samples (reading d_in[i], ld[i]
+ * into a scratch output it allocates), each shape's batch time estimated as
+ * the sum of its beams' times scaled to their full length; then the three
+ * least estimates among the shapes that fit every beam are timed as one
+ * whole launch each into d_out, and the fastest is kept (the next bind uses
+ * it; the buffers are left unbound).  best_variant receives the batch's
+ * variant afterwards, best_ms its launch time. */
 int mdsm_batch_autotune(mdsm_batch batch, const void *const *d_in, const uint64_t *ld,
-                        void *stream, int *best_variant, float *best_ms);
+                        float *const *d_out, void *stream, int *best_variant, float *best_ms);
 /* Launch shape (takes effect at the next bind): FLAT (default) launches one
  * workgroup per tile, numbered so that the DM tiles re-reading one input
  * window share an XCD (and its L2); PERSISTENT runs resident workgroups that

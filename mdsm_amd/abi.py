@@ -120,7 +120,7 @@ _sig("mdsm_batch_launch", _vp, _vp)
 _sig("mdsm_batch_dedisperse", _vp, _P(_vp), _P(_u64), _P(_vp), _vp)
 _sig("mdsm_batch_set_variant", _vp, _i)
 _sig("mdsm_batch_set_mode", _vp, _i)
-_sig("mdsm_batch_autotune", _vp, _P(_vp), _P(_u64), _vp, _P(_i), _P(_f))
+_sig("mdsm_batch_autotune", _vp, _P(_vp), _P(_u64), _P(_vp), _vp, _P(_i), _P(_f))
 BATCH_PERSISTENT, BATCH_FLAT = 0, 1
 _sig("mdsm_generate", C.c_uint64, _u32, C.c_int64, _u64, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp,
      _u64, _vp)
@@ -423,10 +423,11 @@ class Batch:
         self._keep = ((_vp * n)(*d_in), (_u64 * n)(*lds), (_vp * n)(*d_out))
         check(_lib.mdsm_batch_bind(self.handle, *self._keep))
 
-    def autotune(self, d_in, lds, stream=None):
-        """Library tile autotuner; returns (variant, ms of the probe)."""
+    def autotune(self, d_in, lds, d_out, stream=None):
+        """Library tile autotuner; returns (variant, ms of its timed launch).
+        Leaves the batch unbound."""
         n = len(self.surveys)
-        keep = ((_vp * n)(*d_in), (_u64 * n)(*lds))
+        keep = ((_vp * n)(*d_in), (_u64 * n)(*lds), (_vp * n)(*d_out))
         v, ms = _i(0), _f(0.0)
         check(_lib.mdsm_batch_autotune(self.handle, *keep, stream, C.byref(v), C.byref(ms)))
         return v.value, ms.value

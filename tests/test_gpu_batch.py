@@ -59,7 +59,7 @@ def test_batch_uneven_beams_bitexact(dtype):
                 outs[b].upload(np.full(s.ndms * s.nsamp, np.nan, np.float32))
             print(f"[batch dtype {dtype}] mode {mode} variant {v}", flush=True, file=sys.stderr)
             if v == "tuned":  # the library's batch autotuner picks and keeps a shape
-                got_v, ms = batch.autotune([p for p, _ in ins], [l for _, l in ins])
+                got_v, ms = batch.autotune([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
                 assert 1 <= got_v <= abi.tile_variants() and ms > 0
             else:
                 batch.set_variant(v)
