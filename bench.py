@@ -571,8 +571,13 @@ def main():
             ins32.append((d, ld))
         st.free()
         abi.synchronize()
-        # every plain plan gets its own tile shape: the library's plan
-        # autotuner on the beam truncated to 2^16 samples
+        # the same FP32 inputs batched into one launch (batching alone, no
+        # precision change), tuned by the batch autotuner
+        batch32, _, ms32b, B32b, v32b = run_batch(abi.F32, "fp32", ins=ins32)
+        # the plain per-beam path, given its best configuration of two:
+        # every plan with its own tile shape (the library's plan autotuner on
+        # the beam truncated to 2^16 samples), or every plan with the shape
+        # the batch autotuner chose for these FP32 inputs
         v32s = []
         for b, s in enumerate(surveys):
             probe = abi.Survey.make(s.fch1, s.foff, s.nchans, s.tsamp, s.lowdm, s.dmstep, s.ndms,
@@ -580,23 +585,25 @@ def main():
             pp = abi.Plan(probe, abi.F32)
             v32s.append(pp.autotune(ins32[b][0].ptr, ins32[b][1], outs[b].ptr, stream)[0])
             pp.close()
-        for p, v in zip(plans, v32s):
-            p.set_variant(v)
-        v32 = max(set(v32s), key=v32s.count)
 
         def plain_step():
             for b, p in enumerate(plans):
                 p.dedisperse(ins32[b][0].ptr, ins32[b][1], outs[b].ptr, stream)
-        ms32 = time_steps(plain_step, args.steps, args.warmup, stream)
+        tried = {}
+        for name, vs in (("per_beam", v32s), ("batch_shape", [v32b] * len(plans))):
+            for p, v in zip(plans, vs):
+                p.set_variant(v)
+            tried[name] = time_steps(plain_step, args.steps, args.warmup, stream)
+            log(f"[plain fp32] {name} variants {vs}: {tried[name]:.1f} ms/step")
+        choice = min(tried, key=tried.get)
+        ms32 = tried[choice]
         B32 = sum(bytes_per_beam(s, plans[i].maxshift, 4) for i, s in enumerate(surveys))
-        plain = {"ms": ms32, "dm_samples_per_s": total_work / ms32 * 1e3, "tile_variants": v32s,
+        plain = {"ms": ms32, "dm_samples_per_s": total_work / ms32 * 1e3, "configuration": choice,
+                 "ms_per_configuration": tried, "tile_variants": v32s if choice == "per_beam" else [v32b],
                  "hbm_frac": B32 / (ms32 / 1e3) / HBM_PEAK, "launches_per_step": len(plans)}
-        log(f"[plain fp32] variants {v32s}: {ms32:.1f} ms/step, {total_work / ms32 * 1e3:.3e} DM*samples/s")
-        # the same FP32 inputs batched into one launch (batching alone, no
-        # precision change), tuned by the batch autotuner
-        batch32, _, ms32b, B32b, _ = run_batch(abi.F32, "fp32", ins=ins32)
+        log(f"[plain fp32] {choice}: {ms32:.1f} ms/step, {total_work / ms32 * 1e3:.3e} DM*samples/s")
         plain["batched_fp32"] = {"ms": ms32b, "dm_samples_per_s": total_work / ms32b * 1e3,
-                                 "launch_shape": batch32.mode_ms,
+                                 "tile_variant": v32b, "launch_shape": batch32.mode_ms,
                                  "hbm_frac": B32b / (ms32b / 1e3) / HBM_PEAK,
                                  "vs_plain_fp32": ms32 / ms32b}
         del batch32
