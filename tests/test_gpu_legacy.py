@@ -123,3 +123,37 @@ def test_legacy_manager_stream(tmp_path):
     lines = (tmp_path / "legacy_run.dat").read_text().splitlines()
     assert len(lines) == total
     assert len(lines[0].split(", ")) in (3, 5)
+
+
+def test_legacy_manager_file_rotation(tmp_path):
+    """Without singleFileMode the candidates go to
+    <basedir>/<fileprefix>_<time>_<secondsPerFile>.dat, named from the data
+    timestamp (usePCTime 0), and a new file starts once the samples written
+    span more than secondsPerFile (dedispersion_output.cpp:254-281, 296-302)."""
+    xml = tmp_path / "obs.xml"
+    xml.write_text(f'<observation><frequencies top="{FCH1}" offset="{FOFF}"/>'
+                   f'<dm lowDM="{LOWDM}" numDMs="{NDM}" dmStep="{STEP}" useBruteForce="1"/>'
+                   f'<channels number="{NCH}"/><timing tsamp="{TSAMP}"/><samples number="{NSAMP}"/>'
+                   f'<detection threshold="{THR}"/>'
+                   f'<output filePrefix="rot" baseDirectory="{tmp_path}" singleFileMode="0" usePCTime="0" '
+                   'secondsPerFile="1"/></observation>')
+    lib = _lib()
+    assert lib.mdsm_legacy_open(str(xml).encode(), 0, SPC) == 0, lib.mdsm_legacy_last_error()
+    try:
+        info = [C.c_uint() for _ in range(4)]
+        lib.mdsm_legacy_info(*[C.byref(v) for v in info])
+        nch, nsamp, S, _ = (v.value for v in info)
+        buf = np.ctypeslib.as_array(lib.mdsm_legacy_buffer(), shape=(nch, nsamp + S))
+        rng = np.random.default_rng(41)
+        print(f"[legacy rotation] C={nch} N={nsamp} maxshift={S}", file=sys.stderr, flush=True)
+        for k in range(3):
+            buf[:, :] = (rng.standard_normal((nch, nsamp + S)) * 3).astype(np.float32)
+            got = C.c_uint()
+            # 4096 samples x 0.000256 s = 1.05 s per chunk > secondsPerFile: one file per chunk,
+            # named from the chunk's timestamp 1000, 2000, 3000 s
+            lib.mdsm_legacy_next_chunk(nsamp, C.byref(got), 1000.0 * (k + 1), TSAMP, None, None)
+            assert lib.mdsm_legacy_start(nsamp) == k + 1, lib.mdsm_legacy_last_error()
+    finally:
+        lib.mdsm_legacy_close()
+    files = sorted(p.name for p in tmp_path.glob("rot_*_1.dat"))
+    assert len(files) == 3, files
