@@ -68,3 +68,41 @@ def test_batch_uneven_beams_bitexact(dtype):
             for b, s in enumerate(beams):
                 got = outs[b].download(np.float32, (s.ndms, s.nsamp))
                 assert np.array_equal(got, refs[b]), (mode, v, b)
+
+
+@pytest.mark.parametrize("dtype", [abi.F32, abi.F16])
+def test_batch_autotune_skips_shapes_that_do_not_fit(dtype):
+    """A beam with about 1000 samples of delay per DM step (64 channels
+    400 -> 274 MHz, DM step 2.2): the shapes with 64 trials per tile stage
+    rows longer than the LDS holds, so the autotuner must choose among the
+    others; the tuned launch equals the oracle bit for bit."""
+    beams = [abi.Survey.make(400.0, -2.0, 64, 64e-6, 0.0, 2.2, 100, 4096),
+             abi.Survey.make(1500.0, -0.29296875 * 8, 64, 64e-6, 0.0, 0.3, 60, 3000)]
+    batch = abi.Batch(beams, dtype)
+    print(f"[batch autotune, wide spread, dtype {dtype}] maxshift {batch.maxshift}", flush=True,
+          file=sys.stderr)
+    rng = np.random.default_rng(77)
+    ins, outs, refs, keep = [], [], [], []
+    for b, s in enumerate(beams):
+        ld = batch.min_ld[b]
+        x = rng.standard_normal((s.nchans, ld)).astype(np.float32)
+        t, _ = abi.delay_table(s)
+        if dtype == abi.F32:
+            h = x
+            refs.append(O.dedisperse_f32seq(x, t, s.nsamp))
+        else:
+            h = O.f32_to_f16(x)
+            refs.append(O.dedisperse_f16(h, t, s.nsamp))
+        di = abi.DeviceBuffer.from_numpy(h)
+        do = abi.DeviceBuffer(s.ndms * s.nsamp * 4)
+        keep += [di, do]
+        ins.append((di.ptr, ld))
+        outs.append(do)
+    v, ms = batch.autotune([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
+    assert 1 <= v <= abi.tile_variants() and ms > 0
+    for b, s in enumerate(beams):
+        outs[b].upload(np.full(s.ndms * s.nsamp, np.nan, np.float32))
+    batch.dedisperse([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
+    abi.synchronize()
+    for b, s in enumerate(beams):
+        assert np.array_equal(outs[b].download(np.float32, (s.ndms, s.nsamp)), refs[b]), (v, b)
