@@ -328,10 +328,21 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
     // into the raw area (U8); wave w takes 64-dword chunks w, w+4, ...;
     // dwords past the end of the row read as 0
     const uint32_t lane4 = 4u * lane;
+    // the window starts of the rows the next stage() takes, loaded one block
+    // ahead so that their scalar loads do not delay the copies
+    uint32_t g0pre[CB_];
+    auto prefetch = [&](uint32_t cb) {
+#pragma unroll
+      for (int r = 0; r < CB_; ++r)
+        if ((uint32_t)r < cbr && cb + (uint32_t)r < nchans) g0pre[r] = g0_of(cb + (uint32_t)r);
+    };
     auto stage = [&](uint32_t cb, uint32_t *b) {
       const uint32_t nc = min(cbr, nchans - cb);
-      for (uint32_t r = 0; r < nc; ++r) {
-        const uint32_t g0 = g0_of(cb + r);
+#pragma unroll
+      for (int rr = 0; rr < CB_; ++rr) {
+        if ((uint32_t)rr >= nc) break;
+        const uint32_t r = (uint32_t)rr;
+        const uint32_t g0 = g0pre[rr];
         const uint64_t w0 = IS_U8 ? (g0 >> 2) : (g0 >> 1);
         const uint32_t *src = src32 + (uint64_t)(cb + r) * ldw + w0;
         const int32_t ldw32 = (int32_t)ldw;  // rows < 2 GiB
@@ -427,7 +438,9 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
     Rel rel, rel_next;
     const uint32_t nblocks = (nchans + cbr - 1u) / cbr;
     const int32_t *rt = reltab + ((uint64_t)dt * nblocks * 4u + wave) * Rel::WORDS;
+    prefetch(0);
     stage(0, lds);
+    if (cbr < nchans) prefetch(cbr);
     rel.load(rt, lane);
     wait_vmem();
     __syncthreads();
@@ -439,6 +452,7 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
       const bool more = cb + cbr < nchans;
       if (more) {
         stage(cb + cbr, lds + ((k + 1u) & 1u) * bufw);
+        if (cb + 2u * cbr < nchans) prefetch(cb + 2u * cbr);
         rel_next.load(rt + (uint64_t)(k + 1u) * 4u * Rel::WORDS, lane);
       }
       const uint32_t nc = min(cbr, nchans - cb);
