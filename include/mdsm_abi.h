@@ -128,6 +128,12 @@ int mdsm_plan_device_table(mdsm_plan plan, const int32_t **table);
 int mdsm_plan_set_variant(mdsm_plan plan, int variant);
 int mdsm_plan_get_variant(mdsm_plan plan, int *variant);
 int mdsm_tile_variants(int in_dtype, int *count);
+/* The untuned default tile shape of a storage type (a plan uses it when it
+ * can stage the survey's delay spread, else the first shape that can). */
+int mdsm_tile_preferred(int in_dtype, int *variant);
+/* Channels a tile shape stages per LDS block for this plan's delay spread;
+ * 0 when the shape cannot stage it. */
+int mdsm_plan_variant_channels(mdsm_plan plan, int variant, uint32_t *channels);
 /* Time each tile variant on the given buffers (best of three launches after
  * a warm-up) and keep the fastest; the storage type's default variant is
  * kept when it is within 2 % of the fastest. */
@@ -176,13 +182,32 @@ int mdsm_detect(const float *d_x, uint32_t nrows, uint32_t nsamp, uint32_t L,
                 const double *d_mean, const double *d_std, int nwidths, double thresh,
                 mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count, void *stream);
 
-/* Reference-semantics detection (process_brute_clustering,
- * multibeam_dedispersion_output.cpp:206-248): global FP64 mean and rms over
- * all nrows*nsamp values, candidate iff x - mean >= rms*thresh, snr =
- * (x - mean)/rms.  h_mean and h_rms receive the statistics (host). */
-int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, float thresh,
-                       mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count, double *h_mean,
-                       double *h_rms, void *stream);
+/* Reference-semantics detection (process_brute and process_brute_clustering,
+ * multibeam_dedispersion_output.cpp:148-185,188-248) on rows [0, nrows) x
+ * samples [0, nsamp) of d_x with row stride ld (>= nsamp floats): FP64 mean
+ * and rms over the first stats_nsamp samples of every row (stats_nsamp =
+ * nsamp normally; the reference's last buffer takes its statistics over
+ * last_nsamp samples, :216-229), rms = sqrt(S2/n - mean^2); candidate iff
+ * x - mean >= rms*thresh over all nsamp samples, snr = (x - mean)/rms.
+ * The sums run in one fixed order: one partial per 65536-sample chunk of
+ * a row (lane i of 256 sums elements i, i+256, ... of the chunk, then a
+ * binary tree), the partials in row-major chunk order summed the same way;
+ * the reference adds all values in one sequential loop, which rounds
+ * differently in the last bits.  Mean and rms stay on the device for the
+ * threshold; when h_mean or h_rms is given they are copied back and the
+ * stream is synchronised, otherwise the call only enqueues.  Workspace:
+ * nrows*ceil(stats_nsamp/65536) partials, kept by the library per device
+ * and stream (allocated on a stream's first call, grown when needed).
+ * *d_count receives the number of candidates (may exceed cap). */
+int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
+                       uint32_t stats_nsamp, float thresh, mdsm_cand_t *d_cands, uint32_t cap,
+                       uint32_t *d_count, double *h_mean, double *h_rms, void *stream);
+/* The threshold of mdsm_detect_global with the caller's mean and rms: the
+ * reference's process_brute thresholds detrended series at mean 0, rms 1
+ * (multibeam_dedispersion_output.cpp:165-166). */
+int mdsm_threshold_global(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
+                          double mean, double rms, float thresh, mdsm_cand_t *d_cands,
+                          uint32_t cap, uint32_t *d_count, void *stream);
 
 /* Legacy single-beam detector (process_brute,
  * src/mdsm/dedispersion_output.cpp:135-213, one thread's share).  For each

@@ -96,13 +96,16 @@ _sig("mdsm_plan_device_table", _vp, _P(_vp))
 _sig("mdsm_plan_set_variant", _vp, _i)
 _sig("mdsm_plan_get_variant", _vp, _P(_i))
 _sig("mdsm_tile_variants", _i, _P(_i))
+_sig("mdsm_tile_preferred", _i, _P(_i))
+_sig("mdsm_plan_variant_channels", _vp, _i, _P(_u32))
 _sig("mdsm_plan_autotune", _vp, _vp, _u64, _vp, _vp, _P(_i), _P(_f))
 _sig("mdsm_dedisperse", _vp, _vp, _u64, _vp, _vp)
 _sig("mdsm_channel_stats", _vp, _u64, _u32, _u64, _vp, _vp, _vp)
 _sig("mdsm_normalise", _vp, _u64, _u32, _u64, _vp, _vp, _vp, _i, _u64, _vp)
 _sig("mdsm_block_stats", _vp, _u32, _u32, _u32, _vp, _vp, _vp)
 _sig("mdsm_detect", _vp, _u32, _u32, _u32, _vp, _vp, _i, _d, _vp, _u32, _vp, _vp)
-_sig("mdsm_detect_global", _vp, _u32, _u32, _f, _vp, _u32, _vp, _P(_d), _P(_d), _vp)
+_sig("mdsm_detect_global", _vp, _u32, _u32, _u64, _u32, _f, _vp, _u32, _vp, _P(_d), _P(_d), _vp)
+_sig("mdsm_threshold_global", _vp, _u32, _u32, _u64, _d, _d, _f, _vp, _u32, _vp, _vp)
 _sig("mdsm_detect_legacy", _vp, _u32, _u32, _f, _f, _f, _u64, _vp, _vp, _u32, _u32, _vp, _u32, _vp,
      _P(_f), _P(_f), _vp)
 _sig("mdsm_median_filter5", _vp, _u32, _u32, _vp, _vp)
@@ -253,6 +256,12 @@ class Plan:
         check(_lib.mdsm_plan_get_variant(self.handle, C.byref(v)))
         return v.value
 
+    def variant_channels(self, v: int) -> int:
+        """Channels tile shape v stages per LDS block here (0: cannot stage)."""
+        n = _u32(0)
+        check(_lib.mdsm_plan_variant_channels(self.handle, v, C.byref(n)))
+        return n.value
+
     def dedisperse(self, d_in: int, ld: int, d_out: int, stream=None):
         check(_lib.mdsm_dedisperse(self.handle, d_in, ld, d_out, stream))
 
@@ -271,6 +280,12 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def preferred_variant(dtype=F32) -> int:
+    v = _i(0)
+    check(_lib.mdsm_tile_preferred(dtype, C.byref(v)))
+    return v.value
 
 
 def tile_variants(dtype=F32) -> int:
@@ -317,11 +332,20 @@ def detect(d_x, nrows, nsamp, L, d_mean, d_std, nwidths, thresh, d_cands, cap, d
                            d_count, stream))
 
 
-def detect_global(d_x, nrows, nsamp, thresh, d_cands, cap, d_count, stream=None):
+def detect_global(d_x, nrows, nsamp, thresh, d_cands, cap, d_count, stream=None, ld=None,
+                  stats_nsamp=None):
+    """Reference-semantics global detector; returns the (mean, rms) it used."""
     m, r = _d(0.0), _d(0.0)
-    check(_lib.mdsm_detect_global(d_x, nrows, nsamp, thresh, d_cands, cap, d_count, C.byref(m),
-                                  C.byref(r), stream))
+    check(_lib.mdsm_detect_global(d_x, nrows, nsamp, nsamp if ld is None else ld,
+                                  nsamp if stats_nsamp is None else stats_nsamp, thresh, d_cands,
+                                  cap, d_count, C.byref(m), C.byref(r), stream))
     return m.value, r.value
+
+
+def threshold_global(d_x, nrows, nsamp, mean, rms, thresh, d_cands, cap, d_count, stream=None,
+                     ld=None):
+    check(_lib.mdsm_threshold_global(d_x, nrows, nsamp, nsamp if ld is None else ld, mean, rms,
+                                     thresh, d_cands, cap, d_count, stream))
 
 
 def detect_legacy(d_x, ndms, nsamp, lowdm, dmstep, thresh, d_cands, cap, d_count, d_mean=None,

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -113,7 +114,7 @@ using namespace mdsm;
 
 // ======================================================================
 extern "C" const char *mdsm_last_error(void) { return g_err; }
-extern "C" int mdsm_version(void) { return 10000; }
+extern "C" int mdsm_version(void) { return 10100; }
 
 extern "C" int mdsm_device_count(int *count) {
   MDSM_REQUIRE(count, MDSM_ERR_ARG, "null count");
@@ -280,13 +281,19 @@ static int ensure_trial_offsets(mdsm_plan_s *p, int v, hipStream_t s) {
   return MDSM_OK;
 }
 
+// A tile shape can stage this survey when its delay spread has an LDS row
+// and the LDS holds at least one channel of that row.
+static bool variant_fits(const mdsm_plan_s *p, int v) {
+  return p->row_words[v] > 0 && stage_channels(p->dtype, v, p->row_words[v]) > 0;
+}
+
 // Untuned default: the preferred tile shape if it can stage this survey's
 // delay spread, else the first one that can.
 static int default_variant(const mdsm_plan_s *p) {
   const int pref = preferred_variant(p->dtype);
-  if (p->row_words[pref]) return pref;
+  if (variant_fits(p, pref)) return pref;
   for (int v = 1; v < (int)p->row_words.size(); ++v)
-    if (p->row_words[v]) return v;
+    if (variant_fits(p, v)) return v;
   return pref;
 }
 
@@ -371,10 +378,28 @@ extern "C" int mdsm_plan_set_variant(mdsm_plan plan, int variant) {
   // build the variant's trial-offset table now, so that dedisperse calls
   // (and graphs captured from them) allocate nothing
   const int v = plan_variant(plan);
-  if (plan->row_words[v] && stage_channels(plan->dtype, v, plan->row_words[v])) {
+  if (variant_fits(plan, v)) {
     const int rc = ensure_trial_offsets(plan, v, nullptr);
     if (rc != MDSM_OK) return rc;
   }
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_tile_preferred(int in_dtype, int *variant) {
+  MDSM_REQUIRE(variant, MDSM_ERR_ARG, "null variant");
+  MDSM_REQUIRE(in_dtype == MDSM_F32 || in_dtype == MDSM_F16 || in_dtype == MDSM_U8, MDSM_ERR_ARG,
+               "unknown input dtype %d", in_dtype);
+  *variant = preferred_variant(in_dtype);
+  return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_variant_channels(mdsm_plan plan, int variant, uint32_t *channels) {
+  MDSM_REQUIRE(plan && channels, MDSM_ERR_ARG, "null argument");
+  MDSM_REQUIRE(variant >= 1 && variant <= num_variants(plan->dtype), MDSM_ERR_ARG,
+               "variant %d out of range", variant);
+  *channels = variant_fits(plan, variant)
+                  ? stage_channels(plan->dtype, variant, plan->row_words[variant])
+                  : 0u;
   return MDSM_OK;
 }
 
@@ -502,36 +527,111 @@ extern "C" int mdsm_detect(const float *d_x, uint32_t nrows, uint32_t nsamp, uin
                        as_stream(stream));
 }
 
-extern "C" int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, float thresh,
-                                  mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count,
-                                  double *h_mean, double *h_rms, void *stream) {
-  MDSM_REQUIRE(d_x && d_cands && d_count && nrows > 0 && nsamp > 0, MDSM_ERR_ARG, "bad arguments");
-  hipStream_t s = as_stream(stream);
-  const uint32_t nparts = 2048;
-  double *d_part = nullptr;
-  MDSM_HIP(hipMalloc((void **)&d_part, sizeof(double) * 2 * nparts));
-  const uint64_t n = (uint64_t)nrows * nsamp;
-  int rc = launch_global_sums(d_x, n, d_part, nparts, s);
-  std::vector<double> h(2 * nparts);
-  if (rc == MDSM_OK) {
-    MDSM_HIP(hipMemcpyAsync(h.data(), d_part, sizeof(double) * 2 * nparts, hipMemcpyDeviceToHost, s));
-    MDSM_HIP(hipStreamSynchronize(s));
-  }
+namespace mdsm {
+
+// Device workspace of the global detector, one per (device, stream): a call
+// enqueues its kernels on its stream, so calls on one stream reuse the
+// buffer in order and calls on different streams never share one.  Buffers
+// come from the stream-ordered allocator and grow (free + allocate on the
+// same stream) when a larger region arrives.
+namespace {
+struct StreamWorkspace {
+  int device;
+  hipStream_t stream;
+  void *ptr;
+  size_t bytes;
+};
+std::mutex g_ws_mu;
+std::vector<StreamWorkspace> g_ws;
+}  // namespace
+
+int stream_workspace(hipStream_t s, size_t bytes, void **out) {
+  int dev = 0;
+  MDSM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  for (auto &w : g_ws)
+    if (w.device == dev && w.stream == s) {
+      if (w.bytes < bytes) {
+        MDSM_HIP(hipFreeAsync(w.ptr, s));
+        w.ptr = nullptr;
+        w.bytes = 0;
+        MDSM_HIP(hipMallocAsync(&w.ptr, bytes, s));
+        w.bytes = bytes;
+      }
+      *out = w.ptr;
+      return MDSM_OK;
+    }
+  void *p = nullptr;
+  MDSM_HIP(hipMallocAsync(&p, bytes, s));
+  g_ws.push_back({dev, s, p, bytes});
+  *out = p;
+  return MDSM_OK;
+}
+
+// Checks shared by the global detector's entry points.
+int check_global_args(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
+                      uint32_t stats_nsamp, const mdsm_cand_t *d_cands, const uint32_t *d_count) {
+  MDSM_REQUIRE(d_x && d_cands && d_count, MDSM_ERR_ARG, "null argument");
+  MDSM_REQUIRE(nrows > 0 && nrows <= 65535 && nsamp > 0, MDSM_ERR_ARG,
+               "need 1 <= nrows <= 65535 and nsamp > 0");
+  MDSM_REQUIRE(ld >= nsamp, MDSM_ERR_ARG, "row stride %llu < nsamp %u", (unsigned long long)ld,
+               nsamp);
+  MDSM_REQUIRE(stats_nsamp > 0 && stats_nsamp <= nsamp, MDSM_ERR_ARG,
+               "stats_nsamp must be in 1..nsamp");
+  MDSM_REQUIRE(nsamp < (1u << 31), MDSM_ERR_ARG, "nsamp must be < 2^31");
+  return MDSM_OK;
+}
+
+// The global detector's kernels on a given workspace: chunk partials at
+// work[0 .. 2*parts), mean and rms at work[2*parts], work[2*parts + 1].
+int run_global_detect(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
+                      uint32_t stats_nsamp, float thresh, mdsm_cand_t *d_cands, uint32_t cap,
+                      uint32_t *d_count, double *h_mean, double *h_rms, double *work,
+                      hipStream_t s) {
+  const uint32_t parts = global_stats_parts(nrows, stats_nsamp);
+  double *stats = work + 2u * (uint64_t)parts;
+  int rc = launch_global_stats(d_x, ld, nrows, stats_nsamp, work, stats, s);
+  if (rc == MDSM_OK) rc = launch_zero_u32(d_count, 1, s);
+  if (rc == MDSM_OK)
+    rc = launch_threshold_global(d_x, ld, nrows, nsamp, stats, 0.0, 0.0, thresh, d_cands, cap,
+                                 d_count, s);
+  if (rc != MDSM_OK || (!h_mean && !h_rms)) return rc;
+  double h[2];
+  MDSM_HIP(hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
   MDSM_HIP(hipStreamSynchronize(s));
-  MDSM_HIP(hipFree(d_part));
+  if (h_mean) *h_mean = h[0];
+  if (h_rms) *h_rms = h[1];
+  return MDSM_OK;
+}
+
+}  // namespace mdsm
+
+extern "C" int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
+                                  uint32_t stats_nsamp, float thresh, mdsm_cand_t *d_cands,
+                                  uint32_t cap, uint32_t *d_count, double *h_mean, double *h_rms,
+                                  void *stream) {
+  int rc = check_global_args(d_x, nrows, nsamp, ld, stats_nsamp, d_cands, d_count);
   if (rc != MDSM_OK) return rc;
-  double s1 = 0.0, s2 = 0.0;
-  for (uint32_t i = 0; i < nparts; ++i) {
-    s1 += h[2 * i];
-    s2 += h[2 * i + 1];
-  }
-  const double mean = s1 / (double)n;
-  const double rms = std::sqrt(s2 / (double)n - mean * mean);
-  if (h_mean) *h_mean = mean;
-  if (h_rms) *h_rms = rms;
+  hipStream_t s = as_stream(stream);
+  const uint64_t words = 2ull * global_stats_parts(nrows, stats_nsamp) + 2ull;
+  void *work = nullptr;
+  rc = stream_workspace(s, words * sizeof(double), &work);
+  if (rc != MDSM_OK) return rc;
+  return run_global_detect(d_x, nrows, nsamp, ld, stats_nsamp, thresh, d_cands, cap, d_count,
+                           h_mean, h_rms, static_cast<double *>(work), s);
+}
+
+extern "C" int mdsm_threshold_global(const float *d_x, uint32_t nrows, uint32_t nsamp,
+                                     uint64_t ld, double mean, double rms, float thresh,
+                                     mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count,
+                                     void *stream) {
+  int rc = check_global_args(d_x, nrows, nsamp, ld, nsamp, d_cands, d_count);
+  if (rc != MDSM_OK) return rc;
+  hipStream_t s = as_stream(stream);
   rc = launch_zero_u32(d_count, 1, s);
   if (rc != MDSM_OK) return rc;
-  return launch_threshold_global(d_x, nrows, nsamp, mean, rms, thresh, d_cands, cap, d_count, s);
+  return launch_threshold_global(d_x, ld, nrows, nsamp, nullptr, mean, rms, thresh, d_cands, cap,
+                                 d_count, s);
 }
 
 extern "C" int mdsm_detect_legacy(const float *d_x, uint32_t ndms, uint32_t nsamp, float lowdm,

@@ -41,15 +41,23 @@ struct mdsm_batch_s {
   std::vector<BeamDesc> h_desc;
 };
 
+// A tile shape can run the batch when every beam's delay spread has an LDS
+// row (row_words > 0) and the LDS holds at least one staged channel of the
+// widest row (the launch stages every beam with the widest beam's block).
+static bool batch_fits(const mdsm_batch_s *b, int v) {
+  uint32_t maxrw = 0;
+  for (const auto &rw : b->row_words) {
+    if (!rw[v]) return false;
+    maxrw = std::max(maxrw, rw[v]);
+  }
+  return stage_channels(b->dtype, v, maxrw) > 0;
+}
+
 // Untuned default: the preferred tile shape when it fits every beam, else the
 // first shape that does.
 static int batch_variant(const mdsm_batch_s *b) {
   if (b->variant) return b->variant;
-  auto fits = [&](int v) {
-    for (const auto &rw : b->row_words)
-      if (!rw[v]) return false;
-    return true;
-  };
+  auto fits = [&](int v) { return batch_fits(b, v); };
   const int pref = preferred_variant(b->dtype);
   if (fits(pref)) return pref;
   for (int v = 1; v <= num_variants(b->dtype); ++v)
@@ -187,7 +195,7 @@ extern "C" int mdsm_batch_autotune(mdsm_batch batch, const void *const *d_in, co
     mdsm_plan_destroy(p);
     const double scale = (double)batch->beams[i].nsamp / (double)probe.nsamp;
     for (int v = 1; v <= nv; ++v) {
-      if (t.size() <= (size_t)v || t[v] < 0.0f || !batch->row_words[i][v]) ok[v] = false;
+      if (t.size() <= (size_t)v || t[v] < 0.0f || !batch_fits(batch, v)) ok[v] = false;
       else est[v] += scale * t[v];
     }
   }

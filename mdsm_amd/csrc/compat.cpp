@@ -108,8 +108,8 @@ bool detect_beam(unsigned b, const float *d_x, unsigned nsamp_valid, double time
   SURVEY *s = g.survey;
   BeamState &bs = g.beams[b];
   double mean = 0, rms = 0;
-  if (mdsm_detect_global(d_x, g.tdms, nsamp_valid, s->detection_threshold, g.d_cands, g.cap,
-                         g.d_count, &mean, &rms, g.stream) != MDSM_OK) {
+  if (mdsm_detect_global(d_x, g.tdms, nsamp_valid, g.nsamp, nsamp_valid, s->detection_threshold,
+                         g.d_cands, g.cap, g.d_count, &mean, &rms, g.stream) != MDSM_OK) {
     fail("detection failed: %s", mdsm_last_error());
     return false;
   }
@@ -413,20 +413,8 @@ int start_processing(unsigned int data_read) {
       if (!hip_ok(hipMemcpyAsync(g.out_ring[slot][b], bs.d_out, (size_t)g.tdms * N * 4,
                                  hipMemcpyDeviceToHost, g.stream), "output download"))
         return 0;
-      // detection over the valid columns only
-      const float *dx = bs.d_out;
-      float *packed = nullptr;
-      if (data_read < N) {
-        if (!hip_ok(hipMallocAsync((void **)&packed, (size_t)g.tdms * data_read * 4, g.stream), "alloc") ||
-            !hip_ok(hipMemcpy2DAsync(packed, (size_t)data_read * 4, bs.d_out, N * 4,
-                                     (size_t)data_read * 4, g.tdms, hipMemcpyDeviceToDevice, g.stream),
-                    "pack"))
-          return 0;
-        dx = packed;
-      }
-      const bool ok = detect_beam(b, dx, data_read, g.timestamp, g.blockRate);
-      if (packed) (void)hipFreeAsync(packed, g.stream);
-      if (!ok) return 0;
+      // detection over the valid columns only (row stride nsamp)
+      if (!detect_beam(b, bs.d_out, data_read, g.timestamp, g.blockRate)) return 0;
     }
     if (!hip_ok(hipStreamSynchronize(g.stream), "sync")) return 0;
   }

@@ -9,6 +9,9 @@
 // by a CPU restatement of the same order.  These kernels are HBM-bound.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
+#include "detmath.h"
 #include "internal.h"
 
 namespace mdsm {
@@ -240,37 +243,79 @@ __global__ __launch_bounds__(NT) void detect_kernel(const float *__restrict__ x,
 }
 
 // ------------------------------------------------- global sums / threshold
-__global__ __launch_bounds__(NT) void global_sums_kernel(const float *__restrict__ x, uint64_t n,
-                                                         double *__restrict__ partials) {
+// Reference-semantics statistics over the region rows [0, nrows) x columns
+// [0, ncols) of a row-major array with row stride ld, in one fixed order:
+// chunk g = r * nb + b is columns [b * GS_COLS, (b + 1) * GS_COLS) of row r
+// (nb = ceil(ncols / GS_COLS)); workgroup g sums its chunk with the r256
+// pattern (lane i: elements i, i + 256, ... in order; then tree256) into
+// partial g; global_finish_kernel sums the partials with the same pattern
+// (lane i: partials i, i + 256, ...; tree256) and forms
+// mean = S1 / n, rms = sqrt(S2 / n - mean^2) as the reference's
+// process_brute_clustering does (multibeam_dedispersion_output.cpp:206-215).
+__global__ __launch_bounds__(NT) void global_chunk_sums_kernel(const float *__restrict__ x,
+                                                               uint64_t ld, uint32_t ncols,
+                                                               uint32_t nb,
+                                                               double *__restrict__ partials) {
   __shared__ double p1[NT], p2[NT];
-  double a = 0.0, b = 0.0;
-  for (uint64_t j = (uint64_t)blockIdx.x * NT + threadIdx.x; j < n; j += (uint64_t)gridDim.x * NT) {
-    const double v = (double)x[j];
+  const uint32_t g = blockIdx.x;
+  const uint32_t r = g / nb, b = g - r * nb;
+  const uint32_t c0 = b * GS_COLS;
+  const uint32_t n = min(GS_COLS, ncols - c0);
+  const float *row = x + (uint64_t)r * ld + c0;
+  double a = 0.0, q = 0.0;
+  for (uint32_t j = threadIdx.x; j < n; j += NT) {
+    const double v = (double)row[j];
     a = a + v;
-    b = b + v * v;
+    q = q + v * v;
   }
-  tree256(a, b, p1, p2);
+  tree256(a, q, p1, p2);
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = p1[0];
-    partials[2 * blockIdx.x + 1] = p2[0];
+    partials[2u * (uint64_t)g] = p1[0];
+    partials[2u * (uint64_t)g + 1u] = p2[0];
   }
 }
 
+__global__ __launch_bounds__(NT) void global_finish_kernel(const double *__restrict__ partials,
+                                                           uint32_t nparts, double count,
+                                                           double *__restrict__ stats) {
+  __shared__ double p1[NT], p2[NT];
+  double a = 0.0, q = 0.0;
+  for (uint32_t j = threadIdx.x; j < nparts; j += NT) {
+    a = a + partials[2u * (uint64_t)j];
+    q = q + partials[2u * (uint64_t)j + 1u];
+  }
+  tree256(a, q, p1, p2);
+  if (threadIdx.x == 0) {
+    const double mean = p1[0] / count;
+    stats[0] = mean;
+    stats[1] = sqrt(p2[0] / count - mean * mean);
+  }
+}
+
+// Candidate iff x - mean >= rms * thresh in FP64 (thresh a float, as the
+// reference's survey field), snr = (x - mean) / rms; one thread per sample
+// of the rows [0, nrows) x columns [0, ncols) (stride ld), grid (column
+// tiles, rows).  mean and rms come from stats[0..1] on the device, or from
+// the arguments when stats is null.
 __global__ __launch_bounds__(NT) void threshold_global_kernel(const float *__restrict__ x,
-                                                              uint32_t nsamp, uint64_t n,
-                                                              double mean, double rms,
+                                                              uint64_t ld, uint32_t ncols,
+                                                              const double *__restrict__ stats,
+                                                              double mean_arg, double rms_arg,
                                                               float thresh,
                                                               mdsm_cand_t *__restrict__ cands,
                                                               uint32_t cap,
                                                               uint32_t *__restrict__ count) {
+  const double mean = stats ? stats[0] : mean_arg;
+  const double rms = stats ? stats[1] : rms_arg;
   const double lim = rms * (double)thresh;
-  for (uint64_t base = (uint64_t)blockIdx.x * NT; base < n; base += (uint64_t)gridDim.x * NT) {
-    const uint64_t j = base + threadIdx.x;
-    const bool in = j < n;
-    const float v = in ? x[j] : 0.0f;
-    const bool hit = in && ((double)v - mean >= lim);
-    mdsm_cand_t cand{(int32_t)(j / nsamp), (int32_t)(j % nsamp), 0,
-                     (float)(((double)v - mean) / rms)};
+  const uint32_t r = blockIdx.y;
+  const float *row = x + (uint64_t)r * ld;
+  for (uint32_t base = blockIdx.x * NT; base < ncols; base += gridDim.x * NT) {
+    const uint32_t j = base + threadIdx.x;
+    const bool in = j < ncols;
+    const double v = in ? (double)row[j] : 0.0;
+    const bool hit = in && (v - mean >= lim);
+    mdsm_cand_t cand{(int32_t)r, (int32_t)j, 0, (float)((v - mean) / rms)};
     emit(hit, cand, cands, cap, count);
   }
 }
@@ -458,11 +503,13 @@ __global__ __launch_bounds__(NT) void generate_kernel(uint64_t seed, int64_t t0,
        q += (int64_t)gridDim.x * NT) {
     uint32_t ctr[4] = {(uint32_t)q, (uint32_t)((uint64_t)q >> 32), c, 0u};
     philox(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
-    const double two_pi = 6.283185307179586;
-    const double r0 = sqrt(-2.0 * log(u24(ctr[0]))), a0 = two_pi * u24(ctr[1]);
-    const double r1 = sqrt(-2.0 * log(u24(ctr[2]))), a1 = two_pi * u24(ctr[3]);
-    float z[4] = {(float)(r0 * cos(a0)), (float)(r0 * sin(a0)), (float)(r1 * cos(a1)),
-                  (float)(r1 * sin(a1))};
+    // Box-Muller with log and sin/cos from IEEE operations only (detmath.h)
+    const double r0 = sqrt(-2.0 * det_log(u24(ctr[0])));
+    const double r1 = sqrt(-2.0 * det_log(u24(ctr[2])));
+    double s0, c0, s1, c1;
+    det_sincos_turn(u24(ctr[1]), &s0, &c0);
+    det_sincos_turn(u24(ctr[3]), &s1, &c1);
+    float z[4] = {(float)(r0 * c0), (float)(r0 * s0), (float)(r1 * c1), (float)(r1 * s1)};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int64_t t = q * 4 + k;
@@ -534,19 +581,29 @@ int launch_detect(const float *x, uint32_t nrows, uint32_t nsamp, uint32_t L, co
   return MDSM_OK;
 }
 
-int launch_global_sums(const float *x, uint64_t n, double *partials, uint32_t nparts,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(global_sums_kernel, dim3(nparts), dim3(NT), 0, s, x, n, partials);
+uint32_t global_stats_parts(uint32_t nrows, uint32_t ncols) {
+  return nrows * ((ncols + GS_COLS - 1u) / GS_COLS);
+}
+
+int launch_global_stats(const float *x, uint64_t ld, uint32_t nrows, uint32_t ncols,
+                        double *partials, double *stats, hipStream_t s) {
+  const uint32_t nb = (ncols + GS_COLS - 1u) / GS_COLS;
+  const uint32_t nparts = nrows * nb;
+  hipLaunchKernelGGL(global_chunk_sums_kernel, dim3(nparts), dim3(NT), 0, s, x, ld, ncols, nb,
+                     partials);
+  MDSM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(global_finish_kernel, dim3(1), dim3(NT), 0, s, partials, nparts,
+                     (double)nrows * (double)ncols, stats);
   MDSM_HIP(hipGetLastError());
   return MDSM_OK;
 }
 
-int launch_threshold_global(const float *x, uint32_t nrows, uint32_t nsamp, double mean,
-                            double rms, float thresh, mdsm_cand_t *cands, uint32_t cap,
-                            uint32_t *count, hipStream_t s) {
-  const uint64_t n = (uint64_t)nrows * nsamp;
-  hipLaunchKernelGGL(threshold_global_kernel, dim3(grid_for(n, NT * 16, 65536)), dim3(NT), 0, s, x,
-                     nsamp, n, mean, rms, thresh, cands, cap, count);
+int launch_threshold_global(const float *x, uint64_t ld, uint32_t nrows, uint32_t ncols,
+                            const double *stats, double mean, double rms, float thresh,
+                            mdsm_cand_t *cands, uint32_t cap, uint32_t *count, hipStream_t s) {
+  const uint32_t gx = std::min<uint32_t>((ncols + NT * 4u - 1u) / (NT * 4u), 4096u);
+  hipLaunchKernelGGL(threshold_global_kernel, dim3(gx, nrows), dim3(NT), 0, s, x, ld, ncols, stats,
+                     mean, rms, thresh, cands, cap, count);
   MDSM_HIP(hipGetLastError());
   return MDSM_OK;
 }

@@ -108,11 +108,17 @@ int launch_block_stats(const float *x, uint32_t nrows, uint32_t nsamp, uint32_t 
 int launch_detect(const float *x, uint32_t nrows, uint32_t nsamp, uint32_t L, const double *mean,
                   const double *stdv, int nwidths, double thresh, mdsm_cand_t *cands, uint32_t cap,
                   uint32_t *count, hipStream_t s);
-int launch_global_sums(const float *x, uint64_t n, double *partials, uint32_t nparts,
-                       hipStream_t s);
-int launch_threshold_global(const float *x, uint32_t nrows, uint32_t nsamp, double mean,
-                            double rms, float thresh, mdsm_cand_t *cands, uint32_t cap,
-                            uint32_t *count, hipStream_t s);
+// Reference-semantics global statistics: columns per chunk of the fixed
+// reduction order, number of chunk partials for a region, and the two
+// kernels (chunk sums into partials[2 * parts], then mean and rms into
+// stats[2]); the threshold kernel reads mean and rms from stats.
+constexpr uint32_t GS_COLS = 65536;
+uint32_t global_stats_parts(uint32_t nrows, uint32_t ncols);
+int launch_global_stats(const float *x, uint64_t ld, uint32_t nrows, uint32_t ncols,
+                        double *partials, double *stats, hipStream_t s);
+int launch_threshold_global(const float *x, uint64_t ld, uint32_t nrows, uint32_t ncols,
+                            const double *stats, double mean, double rms, float thresh,
+                            mdsm_cand_t *cands, uint32_t cap, uint32_t *count, hipStream_t s);
 int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t s);
 int plan_time_variants(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out,
                        hipStream_t s, std::vector<float> &t);

@@ -3,6 +3,7 @@
 // candidate events, batch gather and class probabilities.
 #include <hip/hip_runtime.h>
 
+#include "detmath.h"
 #include "triage_internal.h"
 
 namespace mdsm {
@@ -52,7 +53,12 @@ __global__ __launch_bounds__(TNT) void rfi_inject_kernel(const RfiItem *__restri
     if (it.kind == RFI_BURST) {
       v = it.amp;
     } else if (it.kind == RFI_MAINS) {
-      v = (float)(it.amp * 0.5 * (1.0 + sin(6.283185307179586 * (double)t / it.period)));
+      // sin(2 pi t / period) from IEEE operations only (detmath.h)
+      double u = (double)t / it.period;
+      u = u - floor(u);
+      double sn, cs;
+      det_sincos_turn(u, &sn, &cs);
+      v = (float)(it.amp * 0.5 * (1.0 + sn));
     } else {  // telegraph: on with probability 1/4 per block of `period` samples
       const uint64_t blk = (uint64_t)t / (uint64_t)it.period;
       const uint32_t r = philox_u32((uint32_t)blk, (uint32_t)(blk >> 32), it.chan + 0x9000u,

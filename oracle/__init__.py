@@ -78,8 +78,11 @@ _sig("oracle_normalise_u8", None, _f32p, _i64, _i, _i64, _f64p, _f64p, _u8p, _i6
 _sig("oracle_block_stats", None, _f64p, _i, _i, _i, _f64p, _f64p)
 _sig("oracle_detect", _i64, _f64p, _i, _i, _i, _f64p, _f64p, _i, _d, _i64,
      _i32p, _i32p, _i32p, _f32p)
-_sig("oracle_global_stats", None, _f32p, _i64, C.POINTER(_d), C.POINTER(_d))
-_sig("oracle_threshold_ref", _i64, _f32p, _i, _i, _d, _d, _f, _i64, _i32p, _i32p, _f32p)
+_sig("oracle_global_stats", None, _f32p, _i64, _i, _i, C.POINTER(_d), C.POINTER(_d))
+_sig("oracle_global_stats_seq", None, _f32p, _i64, _i, _i, C.POINTER(_d), C.POINTER(_d))
+_sig("oracle_threshold_ref", _i64, _f32p, _i64, _i, _i, _d, _d, _f, _i64, _i32p, _i32p, _f32p)
+_sig("oracle_det_log", _d, _d)
+_sig("oracle_det_sincos_turn", None, _d, C.POINTER(_d), C.POINTER(_d))
 _sig("oracle_dbscan2d", _i, _i32p, _i32p, _i, _i, _i, _i32p)
 _sig("oracle_dbscan_ref", _i, _f64p, _f64p, _f64p, _u32, _f, _f, _f, _u32, _i32p)
 _sig("oracle_median5", _f, _f, _f, _f, _f, _f)
@@ -294,23 +297,41 @@ def detect(x, L, nwidths, thresh, cap=1 << 20, stats=None):
     return cd[:k].copy(), ct[:k].copy(), ck[:k].copy(), cs[:k].copy(), int(n)
 
 
-def global_stats(x):
+def global_stats(x, stats_nsamp=None, order="device"):
+    """Reference-semantics (mean, rms) of the 2-D array x over its first
+    stats_nsamp columns; order "device" is the product's fixed chunked
+    order, "sequential" the reference's one loop."""
     x = _c(x, np.float32)
+    nrows, ncols = x.shape
+    sc = ncols if stats_nsamp is None else stats_nsamp
     m = C.c_double()
     r = C.c_double()
-    _lib.oracle_global_stats(x.reshape(-1), x.size, C.byref(m), C.byref(r))
+    fn = _lib.oracle_global_stats if order == "device" else _lib.oracle_global_stats_seq
+    fn(x.reshape(-1), ncols, nrows, sc, C.byref(m), C.byref(r))
     return m.value, r.value
 
 
 def threshold_ref(x, mean, rms, thresh, cap=1 << 20):
+    """Candidates (row, sample, snr float32) of x - mean >= rms*thresh over
+    the whole 2-D array x, in row-major order, and their total count."""
     x = _c(x, np.float32)
     ndms, nsamp = x.shape
     cd = np.empty(cap, np.int32)
     ct = np.empty(cap, np.int32)
     cs = np.empty(cap, np.float32)
-    n = _lib.oracle_threshold_ref(x, ndms, nsamp, mean, rms, thresh, cap, cd, ct, cs)
+    n = _lib.oracle_threshold_ref(x, nsamp, ndms, nsamp, mean, rms, thresh, cap, cd, ct, cs)
     k = min(n, cap)
     return cd[:k].copy(), ct[:k].copy(), cs[:k].copy(), int(n)
+
+
+def det_log(x):
+    return _lib.oracle_det_log(float(x))
+
+
+def det_sincos_turn(u):
+    s, c = C.c_double(), C.c_double()
+    _lib.oracle_det_sincos_turn(float(u), C.byref(s), C.byref(c))
+    return s.value, c.value
 
 
 def dbscan2d(d, t, eps, minpts):

@@ -120,19 +120,66 @@ void oracle_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], ui
 
 static inline double u24(uint32_t x) { return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0); }
 
+/* ln x for normal x > 0 from IEEE +,-,*,/ only (the device generator's
+ * det_log): x = m 2^e, m in [sqrt(1/2), sqrt(2)), ln x = e ln2 + 2 atanh(s),
+ * s = (m-1)/(m+1), odd series of atanh to s^23, Horner from the top term. */
+double oracle_det_log(double x) {
+  uint64_t b;
+  memcpy(&b, &x, 8);
+  int e = (int)((b >> 52) & 0x7FFu) - 1023;
+  uint64_t mb = (b & 0xFFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+  double m;
+  memcpy(&m, &mb, 8);
+  if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
+  double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+  double p = 1.0 / 23.0;
+  for (int k = 21; k >= 3; k -= 2) p = p * s2 + 1.0 / (double)k;
+  p = p * s2 + 1.0;
+  return (double)e * 0.6931471805599453 + 2.0 * (s * p);
+}
+
+/* sin and cos of 2 pi u, 0 <= u <= 1 (the device's det_sincos_turn): the
+ * quarter turn q = (int)(4u + 0.5) is split off, x = 2 pi (u - q/4), Taylor
+ * series of sin to x^17 and cos to x^18 by Horner, then the quadrant. */
+void oracle_det_sincos_turn(double u, double *sn, double *cs) {
+  static const double fact[19] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0,
+                                  362880.0, 3628800.0, 39916800.0, 479001600.0, 6227020800.0,
+                                  87178291200.0, 1307674368000.0, 20922789888000.0,
+                                  355687428096000.0, 6402373705728000.0};
+  int q = (int)(u * 4.0 + 0.5);
+  double r = u - (double)q * 0.25;
+  double x = r * 6.283185307179586, x2 = x * x;
+  /* ps = 1/5! - x^2/7! + ... + x^12/17! */
+  double ps = 1.0 / fact[17];
+  for (int k = 15; k >= 5; k -= 2) ps = ((k / 2) % 2 == 0) ? ps * x2 + 1.0 / fact[k] : ps * x2 - 1.0 / fact[k];
+  double s = (x - (x * x2) * (1.0 / 6.0)) + ((x * x2) * x2) * ps;
+  /* pc = -1/6! + x^2/8! - ... - x^12/18! */
+  double pc = -1.0 / fact[18];
+  for (int k = 16; k >= 6; k -= 2) pc = ((k / 2) % 2 == 0) ? pc * x2 + 1.0 / fact[k] : pc * x2 - 1.0 / fact[k];
+  double c = ((1.0 - x2 * 0.5) + (x2 * x2) * (1.0 / 24.0)) + ((x2 * x2) * x2) * pc;
+  switch (q & 3) {
+    case 0: *sn = s; *cs = c; break;
+    case 1: *sn = c; *cs = -s; break;
+    case 2: *sn = -s; *cs = -c; break;
+    default: *sn = -c; *cs = s; break;
+  }
+}
+
 /* Four N(0,1) draws for channel c, sample quad q (samples 4q..4q+3). */
 static void gauss4(uint64_t seed, uint32_t c, uint64_t q, float z[4]) {
   uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), c, 0u};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   uint32_t o[4];
   oracle_philox4x32_10(ctr, key, o);
-  const double two_pi = 6.283185307179586;
-  double r0 = sqrt(-2.0 * log(u24(o[0]))), a0 = two_pi * u24(o[1]);
-  double r1 = sqrt(-2.0 * log(u24(o[2]))), a1 = two_pi * u24(o[3]);
-  z[0] = (float)(r0 * cos(a0));
-  z[1] = (float)(r0 * sin(a0));
-  z[2] = (float)(r1 * cos(a1));
-  z[3] = (float)(r1 * sin(a1));
+  double r0 = sqrt(-2.0 * oracle_det_log(u24(o[0])));
+  double r1 = sqrt(-2.0 * oracle_det_log(u24(o[2])));
+  double s0, c0, s1, c1;
+  oracle_det_sincos_turn(u24(o[1]), &s0, &c0);
+  oracle_det_sincos_turn(u24(o[3]), &s1, &c1);
+  z[0] = (float)(r0 * c0);
+  z[1] = (float)(r0 * s0);
+  z[2] = (float)(r1 * c1);
+  z[3] = (float)(r1 * s1);
 }
 
 /* Per-channel bandpass gain and offset, drawn from stream 1. */
@@ -587,27 +634,84 @@ int64_t oracle_detect(const double *x, int nrows, int nsamp, int L, const double
 }
 
 /* ------------------------------------------------------------------ */
-/* Reference global statistics + threshold (process_brute_clustering,    */
-/* multibeam_dedispersion_output.cpp:206-248): mean and rms over all      */
-/* M*N values in double, candidate if x - mean >= rms*thresh.             */
+/* Reference global statistics + threshold (process_brute and            */
+/* process_brute_clustering, multibeam_dedispersion_output.cpp:148-185,   */
+/* 188-248): mean and rms in double over the first sc samples of each of  */
+/* the nrows rows (sc = all samples except for the reference's last      */
+/* buffer, :216-229), rms = sqrt(S2/n - mean^2); candidate if             */
+/* x - mean >= rms*thresh over all samples, snr = (x - mean)/rms.         */
 /* ------------------------------------------------------------------ */
-void oracle_global_stats(const float *x, int64_t n, double *mean, double *rms) {
+
+/* The reference's own order: one sequential loop over the values
+ * (row-major), S2 += pow(x, 2) (exact square of a float in double). */
+void oracle_global_stats_seq(const float *x, int64_t ld, int nrows, int sc, double *mean,
+                             double *rms) {
   double m = 0.0, s = 0.0;
-  for (int64_t j = 0; j < n; ++j) {
-    m += x[j];
-    s += pow(x[j], 2);
-  }
-  m /= (double)n;
+  for (int r = 0; r < nrows; ++r)
+    for (int j = 0; j < sc; ++j) {
+      double v = (double)x[(int64_t)r * ld + j];
+      m = m + v;
+      s = s + v * v;
+    }
+  double n = (double)nrows * (double)sc;
+  m = m / n;
   *mean = m;
-  *rms = sqrt(s / (double)n - m * m);
+  *rms = sqrt(s / n - m * m);
 }
 
-int64_t oracle_threshold_ref(const float *x, int ndms, int nsamp, double mean, double rms,
-                             float thresh, int64_t cap, int32_t *cd, int32_t *ct, float *csnr) {
+/* The product's fixed order (mdsm_detect_global): chunk g = r*nb + b holds
+ * samples [b*65536, min((b+1)*65536, sc)) of row r; each chunk is reduced
+ * with the r256 pattern (lane i sums elements i, i+256, ... in order, then
+ * the binary tree s = 128..1), and the chunk partials, in chunk order, are
+ * reduced with the same pattern. */
+#define GS_COLS 65536
+static void r256_pair(const double *a, const double *b, int64_t n, int64_t stride, double *s1,
+                      double *s2) {
+  double p1[R256], p2[R256];
+  for (int i = 0; i < R256; ++i) {
+    double x = 0.0, y = 0.0;
+    for (int64_t j = i; j < n; j += R256) {
+      x = x + a[j * stride];
+      y = y + b[j * stride];
+    }
+    p1[i] = x;
+    p2[i] = y;
+  }
+  for (int s = R256 / 2; s > 0; s >>= 1)
+    for (int i = 0; i < s; ++i) {
+      p1[i] = p1[i] + p1[i + s];
+      p2[i] = p2[i] + p2[i + s];
+    }
+  *s1 = p1[0];
+  *s2 = p2[0];
+}
+
+void oracle_global_stats(const float *x, int64_t ld, int nrows, int sc, double *mean,
+                         double *rms) {
+  int64_t nb = ((int64_t)sc + GS_COLS - 1) / GS_COLS, np = (int64_t)nrows * nb;
+  double *part = (double *)malloc(sizeof(double) * 2 * (size_t)np);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t g = 0; g < np; ++g) {
+    int64_t r = g / nb, b = g % nb, c0 = b * GS_COLS;
+    int64_t n = sc - c0 < GS_COLS ? sc - c0 : GS_COLS;
+    reduce_r256_f(x + r * ld + c0, n, &part[2 * g], &part[2 * g + 1]);
+  }
+  double s1, s2;
+  r256_pair(part, part + 1, np, 2, &s1, &s2);
+  free(part);
+  double n = (double)nrows * (double)sc;
+  double m = s1 / n;
+  *mean = m;
+  *rms = sqrt(s2 / n - m * m);
+}
+
+int64_t oracle_threshold_ref(const float *x, int64_t ld, int nrows, int nsamp, double mean,
+                             double rms, float thresh, int64_t cap, int32_t *cd, int32_t *ct,
+                             float *csnr) {
   int64_t total = 0;
-  for (int k = 0; k < ndms; ++k)
+  for (int k = 0; k < nrows; ++k)
     for (int l = 0; l < nsamp; ++l) {
-      float v = x[(int64_t)k * nsamp + l];
+      float v = x[(int64_t)k * ld + l];
       if (v - mean >= rms * thresh) {
         if (total < cap) { cd[total] = k; ct[total] = l; csnr[total] = (float)((v - mean) / rms); }
         total++;

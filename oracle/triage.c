@@ -17,6 +17,7 @@ typedef struct {
 } rfi_item_t;
 
 void oracle_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]);
+void oracle_det_sincos_turn(double u, double *sn, double *cs);
 
 static float bf16_value(float f) {
   uint32_t u;
@@ -37,7 +38,10 @@ void oracle_rfi_inject(const rfi_item_t *items, int nitems, const float *gain, u
       if (it->kind == 0) {
         v = it->amp;
       } else if (it->kind == 1) {
-        v = (float)(it->amp * 0.5 * (1.0 + sin(6.283185307179586 * (double)t / it->period)));
+        double u = (double)t / it->period, sn, cs;
+        u = u - floor(u);
+        oracle_det_sincos_turn(u, &sn, &cs);
+        v = (float)(it->amp * 0.5 * (1.0 + sn));
       } else {
         const uint64_t blk = (uint64_t)t / (uint64_t)it->period;
         const uint32_t ctr[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), it->chan + 0x9000u, 0u};

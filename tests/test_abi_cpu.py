@@ -139,11 +139,11 @@ def test_xml_legacy_attribute_names():
     assert abs(b.fch1 - 399.951172) < 1e-4 and abs(thr - 4.0) < 1e-6 and mp == 15
 
 
-def test_xml_errors():
+def test_xml_errors(tmp_path):
     beams = (abi.Survey * 1)()
     nb = _C.c_uint32()
     assert _lib.mdsm_compat_parse_xml(b"/nonexistent.xml", beams, 1, _C.byref(nb), None, None, None) != 0
-    bad = os.path.join(ROOT, "_work", "bad.xml") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp/bad.xml"
+    bad = str(tmp_path / "bad.xml")
     with open(bad, "w") as f:
         f.write("<notobservation><dm lowDM='0'/></notobservation>")
     assert _lib.mdsm_compat_parse_xml(bad.encode(), beams, 1, _C.byref(nb), None, None, None) != 0

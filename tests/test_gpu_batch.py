@@ -73,11 +73,18 @@ def test_batch_uneven_beams_bitexact(dtype):
 @pytest.mark.parametrize("dtype", [abi.F32, abi.F16])
 def test_batch_autotune_skips_shapes_that_do_not_fit(dtype):
     """A beam with about 1000 samples of delay per DM step (64 channels
-    400 -> 274 MHz, DM step 2.2): the shapes with 64 trials per tile stage
-    rows longer than the LDS holds, so the autotuner must choose among the
-    others; the tuned launch equals the oracle bit for bit."""
+    400 -> 274 MHz, DM step 2.2): the wide tile shapes, the untuned default
+    among them, stage rows longer than the LDS holds.  The untuned plan and
+    the untuned batch must fall back to a shape that fits, the autotuner
+    must choose among those, and all three equal the oracle bit for bit."""
     beams = [abi.Survey.make(400.0, -2.0, 64, 64e-6, 0.0, 2.2, 100, 4096),
              abi.Survey.make(1500.0, -0.29296875 * 8, 64, 64e-6, 0.0, 0.3, 60, 3000)]
+    # premise: the default shape cannot stage the wide beam
+    wide = abi.Plan(beams[0], dtype)
+    pref = abi.preferred_variant(dtype)
+    assert wide.variant_channels(pref) == 0
+    fallback = wide.variant
+    assert fallback != pref and wide.variant_channels(fallback) > 0
     batch = abi.Batch(beams, dtype)
     print(f"[batch autotune, wide spread, dtype {dtype}] maxshift {batch.maxshift}", flush=True,
           file=sys.stderr)
@@ -98,11 +105,21 @@ def test_batch_autotune_skips_shapes_that_do_not_fit(dtype):
         keep += [di, do]
         ins.append((di.ptr, ld))
         outs.append(do)
-    v, ms = batch.autotune([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
-    assert 1 <= v <= abi.tile_variants() and ms > 0
-    for b, s in enumerate(beams):
-        outs[b].upload(np.full(s.ndms * s.nsamp, np.nan, np.float32))
-    batch.dedisperse([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
+    # (a) the untuned single-beam plan on the wide beam
+    outs[0].upload(np.full(beams[0].ndms * beams[0].nsamp, np.nan, np.float32))
+    wide.dedisperse(ins[0][0], ins[0][1], outs[0].ptr)
     abi.synchronize()
-    for b, s in enumerate(beams):
-        assert np.array_equal(outs[b].download(np.float32, (s.ndms, s.nsamp)), refs[b]), (v, b)
+    assert np.array_equal(outs[0].download(np.float32, (beams[0].ndms, beams[0].nsamp)), refs[0])
+    # (b) the untuned batch, then (c) the autotuned batch
+    for tuned in (False, True):
+        if tuned:
+            v, ms = batch.autotune([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
+            assert 1 <= v <= abi.tile_variants() and ms > 0
+        else:
+            v = "untuned"
+        for b, s in enumerate(beams):
+            outs[b].upload(np.full(s.ndms * s.nsamp, np.nan, np.float32))
+        batch.dedisperse([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
+        abi.synchronize()
+        for b, s in enumerate(beams):
+            assert np.array_equal(outs[b].download(np.float32, (s.ndms, s.nsamp)), refs[b]), (v, b)

@@ -127,17 +127,44 @@ def test_cfg0_pipeline_matches_fp64_oracle():
         assert hit and hit[0]["width_log2"] == 3 and hit[0]["snr"] > 15, (dm, hit)
 
 
-def test_detect_global_reference_semantics():
-    s, t, x, y = _dedispersed(13, 64, 48, 20000, pulses=[(2.0, 3000, 4, 1.0)])
+@pytest.mark.parametrize("case", ["contiguous", "multi_chunk", "strided_prefix"])
+def test_detect_global_reference_semantics(case):
+    """Global FP64 mean/rms and the x - mean >= rms*thr threshold: statistics
+    and candidates (row, sample, snr) bit-exact against the oracle's
+    restatement of the device order."""
+    if case == "contiguous":
+        s, t, x, y = _dedispersed(13, 64, 48, 20000, pulses=[(2.0, 3000, 4, 1.0)])
+        nsamp, ld, sc, thr = 20000, 20000, 20000, 4.0
+    else:
+        rng = np.random.default_rng(21)
+        ncols = 150000 if case == "multi_chunk" else 70001
+        y = (rng.standard_normal((5, ncols)) * 3 + 2).astype(np.float32)
+        y[3, 1234] += 40.0
+        y[4, ncols - 1] += 40.0  # past the thresholded columns in the strided case
+        ld = ncols
+        nsamp = ncols if case == "multi_chunk" else 69000
+        sc = nsamp if case == "multi_chunk" else 66001
+        thr = 4.5
+    nrows = y.shape[0]
+    _say(f"[detect_global] case={case} rows={nrows} nsamp={nsamp} ld={ld} stats={sc}")
     d_y = abi.DeviceBuffer.from_numpy(y)
     cap = 1 << 16
     d_c, d_n = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
-    mean, rms = abi.detect_global(d_y.ptr, 48, 20000, 4.0, d_c.ptr, cap, d_n.ptr)
-    om, orms = O.global_stats(y)
-    assert abs(mean - om) <= 1e-12 * max(1, abs(om)) and abs(rms - orms) <= 1e-12 * orms
+    mean, rms = abi.detect_global(d_y.ptr, nrows, nsamp, thr, d_c.ptr, cap, d_n.ptr, ld=ld,
+                                  stats_nsamp=sc)
+    om, orms = O.global_stats(y, sc) if ld == nsamp else O.global_stats(y[:, :nsamp], sc)
+    assert (mean, rms) == (om, orms)
     c = abi.read_candidates(d_c, d_n, cap)
-    od, ot, osnr, on = O.threshold_ref(y, om, orms, 4.0)
+    od, ot, osnr, on = O.threshold_ref(np.ascontiguousarray(y[:, :nsamp]), om, orms, thr)
+    assert len(c) == on > 0
     assert list(zip(c["dm_idx"], c["sample"])) == list(zip(od, ot))
+    assert np.array_equal(c["snr"], osnr)
+    # the same threshold at caller-given statistics (detrended series: 0 and 1)
+    abi.threshold_global(d_y.ptr, nrows, nsamp, 0.0, 1.0, thr, d_c.ptr, cap, d_n.ptr, ld=ld)
+    c = abi.read_candidates(d_c, d_n, cap)
+    od, ot, osnr, on = O.threshold_ref(np.ascontiguousarray(y[:, :nsamp]), 0.0, 1.0, thr)
+    assert len(c) == on and list(zip(c["dm_idx"], c["sample"])) == list(zip(od, ot))
+    assert np.array_equal(c["snr"], osnr)
 
 
 def _legacy_gpu(x, lowdm, dmstep, thresh, mean=None, rms=None, spc=0, offset=0, nvals=0):
@@ -251,6 +278,5 @@ def test_device_generator_matches_oracle():
     got = d_raw.download(np.float32, (C, n))
     ref = O.generate(9, C, s.fch1, s.foff, s.tsamp, t0, n, g, o,
                      pulses=[(p.dm, p.start, p.width, p.amp) for p in pulses])
-    err = np.abs(got - ref).max()
-    _say(f"[generator] max |gpu - oracle| = {err:.3e}")
-    assert err < 1e-5
+    # Box-Muller from IEEE operations only on both sides: identical bits
+    assert np.array_equal(got, ref), np.abs(got - ref).max()

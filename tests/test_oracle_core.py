@@ -140,7 +140,6 @@ def test_reference_compat_helpers():
     # nsamp: explicit value kept unless below maxshift (then next power of two)
     assert O.calculate_nsamp_ref(65536, 1280, 256, 256, 1000) == 65536
     assert O.calculate_nsamp_ref(1000, 1280, 256, 256, 1000) == 2048
-    assert O.shared_shift_ref(s[-1], 0.0, 0.1, 256, 51.2e-6, 32) > 0
 
 
 # ---------------------------------------------------------------- dedispersion
