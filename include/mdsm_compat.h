@@ -20,7 +20,13 @@
  *    the older number/threshold/ids names) and records which one it used.
  *  - errors are reported by mdsm_compat_last_error() and a 0 / NULL return
  *    instead of exit().
- *  - one GPU: every beam is processed on the device, one after another.
+ *  - one GPU: all beams go through one batched dedispersion launch per
+ *    chunk; uploads, dedispersion and detection + downloads of successive
+ *    chunks overlap on three HIP streams (the reference's three stages).
+ *  - optional FP16 / U8 input storage (<storage type="fp16"/> or "u8"):
+ *    each chunk's rows are bandpass-normalised per channel (mean and
+ *    standard deviation of the row's valid samples) before quantisation,
+ *    so the dedispersed series are sums of normalised samples.
  *  - stream semantics: the first call supplies samples [0, maxshift) of each
  *    channel; every later chunk supplies nsamp new samples, and its
  *    dedispersed output covers the nsamp samples starting maxshift samples
@@ -88,6 +94,10 @@ typedef struct {
   unsigned num_threads;
   unsigned *gpu_ids;
   unsigned num_gpus;
+  /* Extension, after the reference's fields so that their offsets are
+   * unchanged: input storage of the dedisperser, MDSM_F32 (0, the
+   * reference's), MDSM_F16 (1) or MDSM_U8 (2), from <storage type="..."/>. */
+  unsigned input_storage;
 } SURVEY;
 
 /* XML dialect recorded by processSurveyParameters. */
@@ -106,10 +116,14 @@ void tearDownMDSM();
 extern "C" {
 const char *mdsm_compat_last_error(void);
 int mdsm_compat_xml_dialect(void);
-/* Candidates of the last processed chunk of a beam: (time, dm, snr, cluster)
- * arrays owned by the library, valid until the next start_processing. */
+/* Candidates of the last chunk whose detection finished (start_processing(k)
+ * finishes chunk k-1; the first stop call finishes the last chunk) of a
+ * beam: (time, dm, snr, cluster) arrays owned by the library, valid until
+ * the next start_processing. */
 unsigned mdsm_compat_candidates(unsigned beam, const double **time, const double **dm,
                                 const double **snr, const int **cluster);
+/* Input storage in use (MDSM_F32, MDSM_F16 or MDSM_U8). */
+int mdsm_compat_storage(void);
 }
 
 #endif /* MDSM_COMPAT_H */

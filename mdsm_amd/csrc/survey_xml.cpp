@@ -285,6 +285,23 @@ SURVEY *processSurveyParameters(const std::string &filepath) {
       a.u(s->secs_per_file, "secondsPerFile", "outputSecondsPerFile");
       a.c(s->use_pc_time, "usePCTime", "outputUsePCTime");
       a.c(s->single_file_mode, "singleFileMode", "outputSingleFileMode");
+    } else if (n == "storage") {
+      // extension: dedisperser input storage (no reference counterpart, so
+      // it marks neither dialect)
+      auto it = e->attrs.find("type");
+      if (it != e->attrs.end() && !it->second.empty()) {
+        const std::string *v = &it->second;
+        std::string t = *v;
+        for (auto &ch : t) ch = (char)std::tolower((unsigned char)ch);
+        if (t == "fp32" || t == "f32" || t == "float32") s->input_storage = MDSM_F32;
+        else if (t == "fp16" || t == "f16" || t == "float16" || t == "half") s->input_storage = MDSM_F16;
+        else if (t == "u8" || t == "uint8") s->input_storage = MDSM_U8;
+        else {
+          g_xml_err = "unknown storage type: " + *v;
+          free(s);
+          return nullptr;
+        }
+      }
     } else if (n == "gpus") {
       if (auto v = a.get("gpuIDs", "ids")) {
         auto ids = split_uints(*v);

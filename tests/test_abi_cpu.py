@@ -150,3 +150,35 @@ def test_xml_errors(tmp_path):
     with open(bad, "w") as f:
         f.write("<observation><dm lowDM='0' </observation>")
     assert _lib.mdsm_compat_parse_xml(bad.encode(), beams, 1, _C.byref(nb), None, None, None) != 0
+
+
+_lib.mdsm_compat_open.argtypes = [_C.c_char_p, _C.c_uint]
+_lib.mdsm_compat_open.restype = _C.c_int
+_lib.mdsm_compat_last_error.restype = _C.c_char_p
+
+_XML_DIR = """<observation>
+  <dm lowDM="0" numDMs="16" dmStep="1.0" />
+  <channels nchans="32" npols="1" />
+  <timing tsamp="0.000064" />
+  <samples nsamp="4096" nbits="32" />
+  {storage}
+  <writer filePrefix="x" baseDirectory="{dir}" singleFileMode="1" />
+  <beams><beam beamId="0" topFrequency="1500" frequencyOffset="-1.0" /></beams>
+</observation>"""
+
+
+def test_compat_unwritable_output_directory_fails_initialisation(tmp_path):
+    """The reference exits when a candidate file cannot be created
+    (multibeam_dedispersion_output.cpp:41-45); here initialiseMDSM fails
+    (before any device call) and reports the directory."""
+    xml = tmp_path / "obs.xml"
+    xml.write_text(_XML_DIR.format(dir=str(tmp_path / "missing" / "dir"), storage=""))
+    assert _lib.mdsm_compat_open(str(xml).encode(), 0) == 2
+    assert b"Invalid output directory" in _lib.mdsm_compat_last_error()
+
+
+def test_compat_storage_element(tmp_path):
+    xml = tmp_path / "obs.xml"
+    xml.write_text(_XML_DIR.format(dir=str(tmp_path), storage='<storage type="bogus" />'))
+    assert _lib.mdsm_compat_open(str(xml).encode(), 0) == 1
+    assert b"unknown storage type" in _lib.mdsm_compat_last_error()
