@@ -311,7 +311,7 @@ def _check_cands(cands, series, ts, block, stats_cols=None, mean_rms=None, thr=5
     else:
         mean, rms = mean_rms
     od, ot, osnr, on = O.threshold_ref(series, mean, rms, thr)
-    assert len(t) == on > 0
+    assert len(t) == on and on > 0, (len(t), on)
     assert np.array_equal(np.rint((t - ts) / block).astype(int), ot)
     assert np.array_equal(d, od.astype(np.float32).astype(np.float64))
     assert np.array_equal(s.astype(np.float32), osnr)
@@ -323,7 +323,8 @@ def test_streaming_manager_short_last_buffer(tmp_path):
     reads = [N, N, 1500]
     total = S + sum(reads)
     stream = O.generate(31, C_, 1500.0, -1.0, 64e-6, 0, total,
-                        pulses=[(20.0, 3000, 8, 1.2), (25.0, S + 2 * N + 200, 8, 1.5)])
+                        pulses=[(20.0, 3000, 8, 1.2), (17.0, N + 1900, 8, 1.5),
+                                (25.0, 2 * N + 200, 8, 1.5)])
     print(f"[compat short] C={C_} N={N} M={M} S={S} reads={reads}", file=sys.stderr, flush=True)
     got, cands = _drive(stream, C_, S, N, M, reads, ts=lambda i: 10.0 * i)
     lib.mdsm_compat_close()
@@ -347,7 +348,7 @@ def test_streaming_manager_last_nsamp_statistics(tmp_path):
     C_, N, M, S = _open(tmp_path, _plain())
     table = O.delay_table_ref(1500.0, -1.0, C_, 64e-6, 0.0, 1.0, M)
     reads = [N, N]
-    stream = O.generate(32, C_, 1500.0, -1.0, 64e-6, 0, S + 2 * N, pulses=[(22.0, S + N + 900, 8, 1.5)])
+    stream = O.generate(32, C_, 1500.0, -1.0, 64e-6, 0, S + 2 * N, pulses=[(19.0, 1300, 8, 1.5), (22.0, N + 900, 8, 1.5)])
     last = 2100
     got, cands = _drive(stream, C_, S, N, M, reads,
                         before_stop=lambda: lib.mdsm_compat_set_last_nsamp(last))
@@ -363,7 +364,7 @@ def test_streaming_manager_detrended_threshold(tmp_path):
     multibeam_dedispersion_output.cpp:165-166), checked on the series the
     manager returns."""
     C_, N, M, S = _open(tmp_path, _plain(detect='applyDetrending="1"'))
-    stream = O.generate(33, C_, 1500.0, -1.0, 64e-6, 0, S + 2 * N, pulses=[(15.0, S + 700, 8, 1.5)])
+    stream = O.generate(33, C_, 1500.0, -1.0, 64e-6, 0, S + 2 * N, pulses=[(15.0, 700, 8, 1.5), (27.0, N + 2500, 8, 1.5)])
     got, cands = _drive(stream, C_, S, N, M, [N, N])
     lib.mdsm_compat_close()
     for k in (1, 2):
@@ -394,8 +395,7 @@ def test_streaming_manager_file_rotation(tmp_path):
     (multibeam_dedispersion_output.cpp:21-86,403-406)."""
     C_, N, M, S = _open(tmp_path, _plain(single=0, spf=0))
     stream = O.generate(35, C_, 1500.0, -1.0, 64e-6, 0, S + 3 * N,
-                        pulses=[(12.0, S + 500, 8, 2.0), (12.0, S + N + 500, 8, 2.0),
-                                (12.0, S + 2 * N + 500, 8, 2.0)])
+                        pulses=[(12.0, 500, 8, 2.0), (12.0, N + 500, 8, 2.0), (12.0, 2 * N + 500, 8, 2.0)])
     base = 1.7e9
     got, cands = _drive(stream, C_, S, N, M, [N, N, N], ts=lambda i: base + 100.0 * i)
     lib.mdsm_compat_close()
@@ -423,7 +423,8 @@ def test_streaming_manager_reduced_storage(tmp_path, storage):
     gain, offset = O.channel_bandpass(36, C_)
     reads = [N, N, 2000]
     stream = O.generate(36, C_, 1500.0, -1.0, 64e-6, 0, S + sum(reads), gain=gain, offset=offset,
-                        pulses=[(18.0, S + 1200, 8, 2.0), (30.0, S + 2 * N + 300, 4, 2.5)])
+                        pulses=[(18.0, 1200, 8, 2.0), (24.0, N + 3000, 8, 2.0),
+                                (30.0, 2 * N + 300, 4, 2.5)])
     got, cands = _drive(stream, C_, S, N, M, reads)
     lib.mdsm_compat_close()
     ld = (N + S + 3) // 4 * 4
@@ -434,7 +435,7 @@ def test_streaming_manager_reduced_storage(tmp_path, storage):
         row[:, S:S + n] = stream[:, col:col + n]
         col += n
         m, sd = O.channel_stats(row, S + n)
-        q = O.normalise(row, m, sd, storage)
+        q = O.normalise(row, m, sd, {"fp16": "f16", "u8": "u8"}[storage])
         want = O.dedisperse_f16(q, table, N) if storage == "fp16" else O.dedisperse_u8(q, table, N)
         assert np.array_equal(got[k][:, :n], want[:, :n]), k
         _check_cands(cands[k], np.ascontiguousarray(want[:, :n]), 0.0, 64e-6)

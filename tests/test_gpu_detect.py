@@ -159,10 +159,12 @@ def test_detect_global_reference_semantics(case):
     assert len(c) == on > 0
     assert list(zip(c["dm_idx"], c["sample"])) == list(zip(od, ot))
     assert np.array_equal(c["snr"], osnr)
-    # the same threshold at caller-given statistics (detrended series: 0 and 1)
-    abi.threshold_global(d_y.ptr, nrows, nsamp, 0.0, 1.0, thr, d_c.ptr, cap, d_n.ptr, ld=ld)
+    # the same threshold at caller-given statistics
+    fm, fr = om + 0.25 * orms, 1.125 * orms
+    abi.threshold_global(d_y.ptr, nrows, nsamp, fm, fr, thr, d_c.ptr, cap, d_n.ptr, ld=ld)
     c = abi.read_candidates(d_c, d_n, cap)
-    od, ot, osnr, on = O.threshold_ref(np.ascontiguousarray(y[:, :nsamp]), 0.0, 1.0, thr)
+    od, ot, osnr, on = O.threshold_ref(np.ascontiguousarray(y[:, :nsamp]), fm, fr, thr)
+    assert on > 0
     assert len(c) == on and list(zip(c["dm_idx"], c["sample"])) == list(zip(od, ot))
     assert np.array_equal(c["snr"], osnr)
 
