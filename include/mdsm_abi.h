@@ -202,6 +202,20 @@ int mdsm_detect(const float *d_x, uint32_t nrows, uint32_t nsamp, uint32_t L,
 int mdsm_detect_global(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
                        uint32_t stats_nsamp, float thresh, mdsm_cand_t *d_cands, uint32_t cap,
                        uint32_t *d_count, double *h_mean, double *h_rms, void *stream);
+/* mdsm_detect_global without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a fixed 2 MiB device array per GPU instead of the library's
+ * per-stream buffers, so the region is limited to
+ * nrows * ceil(stats_nsamp / 65536) <= 131072 chunk partials (e.g. 4096 rows
+ * of up to 2^21 samples); larger regions are rejected with MDSM_ERR_ARG.
+ * Every call on a GPU uses that GPU's one array: calls that overlap on one
+ * GPU (other streams or threads) race, so callers serialise them.  Same
+ * arguments, order of sums and results as mdsm_detect_global. */
+int mdsm_detect_global_noalloc(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
+                               uint32_t stats_nsamp, float thresh, mdsm_cand_t *d_cands,
+                               uint32_t cap, uint32_t *d_count, double *h_mean, double *h_rms,
+                               void *stream);
 /* The threshold of mdsm_detect_global with the caller's mean and rms: the
  * reference's process_brute thresholds detrended series at mean 0, rms 1
  * (multibeam_dedispersion_output.cpp:165-166). */

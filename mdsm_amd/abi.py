@@ -106,6 +106,8 @@ _sig("mdsm_block_stats", _vp, _u32, _u32, _u32, _vp, _vp, _vp)
 _sig("mdsm_detect", _vp, _u32, _u32, _u32, _vp, _vp, _i, _d, _vp, _u32, _vp, _vp)
 _sig("mdsm_detect_global", _vp, _u32, _u32, _u64, _u32, _f, _vp, _u32, _vp, _P(_d), _P(_d), _vp)
 _sig("mdsm_threshold_global", _vp, _u32, _u32, _u64, _d, _d, _f, _vp, _u32, _vp, _vp)
+_sig("mdsm_detect_global_noalloc", _vp, _u32, _u32, _u64, _u32, _f, _vp, _u32, _vp, _P(_d), _P(_d),
+     _vp)
 _sig("mdsm_detect_legacy", _vp, _u32, _u32, _f, _f, _f, _u64, _vp, _vp, _u32, _u32, _vp, _u32, _vp,
      _P(_f), _P(_f), _vp)
 _sig("mdsm_median_filter5", _vp, _u32, _u32, _vp, _vp)
@@ -339,6 +341,17 @@ def detect_global(d_x, nrows, nsamp, thresh, d_cands, cap, d_count, stream=None,
     check(_lib.mdsm_detect_global(d_x, nrows, nsamp, nsamp if ld is None else ld,
                                   nsamp if stats_nsamp is None else stats_nsamp, thresh, d_cands,
                                   cap, d_count, C.byref(m), C.byref(r), stream))
+    return m.value, r.value
+
+
+def detect_global_noalloc(d_x, nrows, nsamp, thresh, d_cands, cap, d_count, stream=None, ld=None,
+                          stats_nsamp=None):
+    """mdsm_detect_global on the library's fixed device workspace (no
+    allocation); returns the (mean, rms) it used."""
+    m, r = _d(0.0), _d(0.0)
+    check(_lib.mdsm_detect_global_noalloc(d_x, nrows, nsamp, nsamp if ld is None else ld,
+                                          nsamp if stats_nsamp is None else stats_nsamp, thresh,
+                                          d_cands, cap, d_count, C.byref(m), C.byref(r), stream))
     return m.value, r.value
 
 

@@ -10,6 +10,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "detmath.h"
 #include "internal.h"
@@ -663,3 +664,61 @@ int launch_generate(uint64_t seed, uint32_t nchans, int64_t t0, uint64_t n, cons
 }
 
 }  // namespace mdsm
+
+// ------------------------------------------- allocation-free global detector
+// Workspace of mdsm_detect_global_noalloc: a fixed device array, so that the
+// call needs no allocation (for callers under graph capture or in loops that
+// must not allocate).  Every call on a device uses that device's copy and
+// leaves its last contents there; calls that overlap on one device race.
+// Room for SCRATCH_PARTS chunk partials (each 65536 samples of one row: at
+// most nrows * ceil(stats_nsamp / 65536) = 131072, e.g. 4096 rows of 2^21
+// samples), then mean and rms; 2 MiB per device.
+namespace {
+constexpr uint32_t SCRATCH_PARTS = 131072;
+static __device__ __align__(256) double scratch[2 * SCRATCH_PARTS + 2];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's scratch, looked up once per device
+// with the device current (the array's name in host code is only a host-side
+// stand-in, never a device address).
+int scratch_address(double **out) {
+  int dev = 0;
+  MDSM_HIP(hipGetDevice(&dev));
+  MDSM_REQUIRE(dev >= 0 && dev < kMaxDevices, MDSM_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      mdsm::set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return MDSM_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MDSM_OK;
+}
+}  // namespace
+
+extern "C" int mdsm_detect_global_noalloc(const float *d_x, uint32_t nrows, uint32_t nsamp,
+                                          uint64_t ld, uint32_t stats_nsamp, float thresh,
+                                          mdsm_cand_t *d_cands, uint32_t cap, uint32_t *d_count,
+                                          double *h_mean, double *h_rms, void *stream) {
+  using namespace mdsm;
+  int rc = check_global_args(d_x, nrows, nsamp, ld, stats_nsamp, d_cands, d_count);
+  if (rc != MDSM_OK) return rc;
+  const uint32_t parts = global_stats_parts(nrows, stats_nsamp);
+  MDSM_REQUIRE(parts <= SCRATCH_PARTS, MDSM_ERR_ARG,
+               "region of %u rows x %u samples needs %u partials; the static workspace holds %u "
+               "(rows * ceil(stats_nsamp / 65536) <= %u)",
+               nrows, stats_nsamp, parts, SCRATCH_PARTS, SCRATCH_PARTS);
+  double *work = nullptr;
+  rc = scratch_address(&work);
+  if (rc != MDSM_OK) return rc;
+  return run_global_detect(d_x, nrows, nsamp, ld, stats_nsamp, thresh, d_cands, cap, d_count,
+                           h_mean, h_rms, work, as_stream(stream));
+}
+

@@ -120,6 +120,14 @@ int launch_threshold_global(const float *x, uint64_t ld, uint32_t nrows, uint32_
                             const double *stats, double mean, double rms, float thresh,
                             mdsm_cand_t *cands, uint32_t cap, uint32_t *count, hipStream_t s);
 int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t s);
+// The global detector's argument checks and kernels on a given workspace
+// (partials at work[0 .. 2*parts), mean and rms after them).
+int check_global_args(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
+                      uint32_t stats_nsamp, const mdsm_cand_t *d_cands, const uint32_t *d_count);
+int run_global_detect(const float *d_x, uint32_t nrows, uint32_t nsamp, uint64_t ld,
+                      uint32_t stats_nsamp, float thresh, mdsm_cand_t *d_cands, uint32_t cap,
+                      uint32_t *d_count, double *h_mean, double *h_rms, double *work,
+                      hipStream_t s);
 int plan_time_variants(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out,
                        hipStream_t s, std::vector<float> &t);
 int pick_variant(int dtype, const std::vector<float> &t);
