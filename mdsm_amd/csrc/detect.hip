@@ -243,6 +243,121 @@ __global__ __launch_bounds__(NT) void detect_kernel(const float *__restrict__ x,
   }
 }
 
+// Register-resident detector (block lengths that are multiples of 16): each
+// lane holds DW_R = 16 consecutive samples of one DM row and builds every
+// boxcar level in place in FP64 registers,
+// S_k(i) = S_{k-1}(i) + S_{k-1}(i + h), h = 2^(k-1); the terms past the
+// lane's own run come from the next lane (h <= 16) or the lane after it
+// (h = 32) by wave shuffles, so a level costs no LDS round trip and no
+// barrier.  A wave covers 64 runs; after seven levels the runs of lanes
+// 0..56 are complete, so a wave owns DW_SEG = 57 * 16 samples and reads the
+// next 112 again (12 % extra reads, mostly L2 hits).  Each run lies inside
+// one statistics block (L % 16 == 0), so a lane forms its block's w * mean
+// and threshold bounds once per level instead of once per sample.  Same
+// sums, same S/N arithmetic and same pass bound as detect_kernel, so the
+// candidates and S/N values are identical.
+constexpr int DW_R = 16;
+constexpr int DW_LANES = 57;
+constexpr int DW_SEG = DW_LANES * DW_R;
+
+__global__ __launch_bounds__(NT) void detect_wave_kernel(
+    const float *__restrict__ x, uint32_t nsamp, uint32_t L, uint32_t nb,
+    const double *__restrict__ mean, const double *__restrict__ stdv, int nwidths, double thresh,
+    mdsm_cand_t *__restrict__ cands, uint32_t cap, uint32_t *__restrict__ count) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t d = blockIdx.y;
+  const uint64_t seg0 = ((uint64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6)) * DW_SEG;
+  if (seg0 >= nsamp) return;  // whole wave; this kernel has no barriers
+  const uint64_t t0 = seg0 + (uint64_t)DW_R * lane;
+  // the row through a buffer resource bounded to nsamp floats: samples past
+  // the row end read as 0 (they only feed levels that run past nsamp)
+  const float *row = x + (uint64_t)d * nsamp;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(row), (short)0, (int)(4u * nsamp), 0x00020000);
+  double v[DW_R];
+  const int byte0 = (int)min(t0, (uint64_t)nsamp) * 4;
+  // one range-checked dword per sample (no branch on the row end: the
+  // hardware returns 0 past num_records)
+#pragma unroll
+  for (int i = 0; i < DW_R; ++i)
+    v[i] = (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, byte0 + 4 * i, 0, 0));
+
+  // the run's block statistics and per-level bounds (see detect_kernel for
+  // the pass bound a >= lim * den)
+  const uint32_t b = (uint32_t)(min(t0, (uint64_t)nsamp - 1u) / L);
+  const double m = mean[(uint64_t)d * nb + b], sd = stdv[(uint64_t)d * nb + b];
+  const bool live = sd > 0.0;
+  const double lim = thresh > 0.0 ? thresh * (1.0 - 0x1p-40) : -INFINITY;
+  // samples of the run whose level-k window ends inside the row: i < n_in_k
+  const int64_t room = (int64_t)nsamp - (int64_t)t0;  // samples from t0 to the row end
+  double best[DW_R];
+  uint32_t bk = 0xFFFFFFFFu;  // level of best, 4 bits per sample (>= 8: none)
+  uint32_t bk2 = 0xFFFFFFFFu;
+#pragma unroll
+  for (int i = 0; i < DW_R; ++i) best[i] = -INFINITY;
+  // Level k: a = S_k - w * mean for all 16 samples and one test of the
+  // whole run against the pass bound; only a run with a passing sample takes
+  // the (rare) branch that checks the window against the row end, divides
+  // and keeps the best level.
+  auto test_level = [&](int k) {
+    const double w = (double)(1u << k);
+    const double wm = w * m;
+    const double den = sd * kSqrtW[k];
+    const bool in_range = den > 0x1p-960 && den < 0x1p960;
+    const double bound = in_range ? lim * den : -INFINITY;
+    double a[DW_R];
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < DW_R; ++i) {
+      a[i] = v[i] - wm;
+      any |= a[i] >= bound;
+    }
+    if (live && any) {
+#pragma unroll
+      for (int i = 0; i < DW_R; ++i) {
+        if ((int64_t)i + (int64_t)(1u << k) <= room && a[i] >= bound) {
+          const double snr = a[i] / den;
+          if (snr > best[i]) {
+            best[i] = snr;
+            if (i < 8) bk = (bk & ~(15u << (4 * i))) | ((uint32_t)k << (4 * i));
+            else bk2 = (bk2 & ~(15u << (4 * (i - 8)))) | ((uint32_t)k << (4 * (i - 8)));
+          }
+        }
+      }
+    }
+  };
+  test_level(0);
+#pragma unroll
+  for (int k = 1; k < 7; ++k) {
+    if (k >= nwidths) break;
+    const int h = 1 << (k - 1);
+    double nx[DW_R];
+    if (h < DW_R) {
+      // the first h values of the next lane
+#pragma unroll
+      for (int j = 0; j < DW_R; ++j)
+        if (j < h) nx[j] = __shfl_down(v[j], 1);
+#pragma unroll
+      for (int i = 0; i < DW_R; ++i) v[i] = v[i] + (i + h < DW_R ? v[i + h] : nx[i + h - DW_R]);
+    } else {
+      // h = 16: all of the next lane; h = 32: all of the lane after it
+#pragma unroll
+      for (int j = 0; j < DW_R; ++j) nx[j] = __shfl_down(v[j], h / DW_R);
+#pragma unroll
+      for (int i = 0; i < DW_R; ++i) v[i] = v[i] + nx[i];
+    }
+    test_level(k);
+  }
+  const bool own = lane < (uint32_t)DW_LANES;
+#pragma unroll
+  for (int i = 0; i < DW_R; ++i) {
+    const uint32_t k = i < 8 ? (bk >> (4 * i)) & 15u : (bk2 >> (4 * (i - 8))) & 15u;
+    const bool hit = own && k < 8u && best[i] >= thresh;
+    mdsm_cand_t cand{(int32_t)d, (int32_t)(t0 + (uint64_t)i), (int32_t)k, (float)best[i]};
+    emit(hit, cand, cands, cap, count);
+  }
+}
+
 // ------------------------------------------------- global sums / threshold
 // Reference-semantics statistics over the region rows [0, nrows) x columns
 // [0, ncols) of a row-major array with row stride ld, in one fixed order:
@@ -575,6 +690,15 @@ int launch_detect(const float *x, uint32_t nrows, uint32_t nsamp, uint32_t L, co
                   const double *stdv, int nwidths, double thresh, mdsm_cand_t *cands, uint32_t cap,
                   uint32_t *count, hipStream_t s) {
   const uint32_t nb = (nsamp + L - 1) / L;
+  if (L % DW_R == 0 && nsamp < (1u << 29)) {
+    // register-resident form: four waves of DW_SEG samples per workgroup
+    const uint64_t per_wg = (uint64_t)DW_SEG * (NT / 64);
+    const uint32_t nwg = (uint32_t)(((uint64_t)nsamp + per_wg - 1u) / per_wg);
+    hipLaunchKernelGGL(detect_wave_kernel, dim3(nwg, nrows), dim3(NT), 0, s, x, nsamp, L, nb, mean,
+                       stdv, nwidths, thresh, cands, cap, count);
+    MDSM_HIP(hipGetLastError());
+    return MDSM_OK;
+  }
   const uint32_t ntile = (nsamp + DT_T - 1) / DT_T;
   hipLaunchKernelGGL(detect_kernel, dim3(ntile, nrows), dim3(NT), 0, s, x, nsamp, L, nb, mean, stdv,
                      nwidths, thresh, cands, cap, count);

@@ -75,6 +75,34 @@ def test_block_stats_and_detect_bitexact(L):
         assert np.array_equal(c["snr"], osnr)
 
 
+@pytest.mark.parametrize("N,L,nw", [(40000, 2048, 4), (913, 16, 7), (100, 64, 7), (1, 16, 3),
+                                     (5001, 5000, 7), (70000, 8192, 1)])
+def test_detect_row_tails_and_widths(N, L, nw):
+    """Rows whose length is not a multiple of the detector's 912-sample wave
+    segment or 16-sample runs, shorter than a run or a block, fewer boxcar
+    widths, and block lengths on both detector paths (multiples of 16 use
+    the register-resident kernel, others the LDS-tiled one)."""
+    rng = np.random.default_rng(N + L + nw)
+    M = 7
+    y = rng.standard_normal((M, N)).astype(np.float32)
+    y[:, N // 2: N // 2 + 8] += 3.0                      # a boxcar for the wider levels
+    y[:, -1] += 9.0                                       # the row's last sample
+    d_y = abi.DeviceBuffer.from_numpy(y)
+    nb = (N + L - 1) // L
+    d_m, d_s = abi.DeviceBuffer(M * nb * 8), abi.DeviceBuffer(M * nb * 8)
+    abi.block_stats(d_y.ptr, M, N, L, d_m.ptr, d_s.ptr)
+    om, os_ = O.block_stats(y.astype(np.float64), L)
+    cap = 1 << 16
+    d_c, d_n = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+    for thr in (2.5, 4.0):
+        abi.detect(d_y.ptr, M, N, L, d_m.ptr, d_s.ptr, nw, thr, d_c.ptr, cap, d_n.ptr)
+        c = abi.read_candidates(d_c, d_n, cap)
+        od, ot, ok, osnr, on = O.detect(y.astype(np.float64), L, nw, thr, stats=(om, os_))
+        assert len(c) == on
+        assert _cands_tuple(c) == list(zip(od.tolist(), ot.tolist(), ok.tolist()))
+        assert np.array_equal(c["snr"], osnr)
+
+
 def test_detect_overflow_is_reported():
     s, t, x, y = _dedispersed(12, 64, 32, 8192)
     d_y = abi.DeviceBuffer.from_numpy(y)
