@@ -182,11 +182,30 @@ def detect_beam(beam, d_out, stream, L=16384, thr=6.0, cap=1 << 21):
     return len(c), nc, found
 
 
-def cpu_baseline(target_s=12.0):
+def cpu_model():
+    """CPU model name and the cores this process may use (Linux)."""
+    name = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return name, avail
+
+
+def cpu_baseline(run_s=2.5, runs=5):
     """The oracle's multithreaded FP32 channel-order brute force (the
     reference's arithmetic, OpenMP over DM x time tiles, SIMD over time) on a
     bounded sample of the single-beam survey: 1024 channels x 2^17 samples x
-    D DM trials (step 0.2, D <= 4096), D sized for about target_s seconds."""
+    D DM trials (step 0.2, D <= 4096), D sized for about run_s seconds per
+    run; the median of `runs` runs is reported, with every run's rate."""
     import oracle as O
     N = 1 << 17
     s = abi.Survey.make(FCH1, -BAND / 1024, 1024, TSAMP, 0.0, 0.2, 4096, N)
@@ -196,28 +215,38 @@ def cpu_baseline(target_s=12.0):
     t0 = time.perf_counter()
     O.dedisperse_f32seq(x, np.ascontiguousarray(table[:, :d0]), N)
     el = time.perf_counter() - t0
-    D = int(min(4096, max(d0, d0 * target_s / max(el, 1e-3))))
+    D = int(min(4096, max(d0, d0 * run_s / max(el, 1e-3))))
     D = max(64, (D // 64) * 64)
     tab = np.ascontiguousarray(table[:, :D])
-    t0 = time.perf_counter()
-    O.dedisperse_f32seq(x, tab, N)
-    el = time.perf_counter() - t0
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        O.dedisperse_f32seq(x, tab, N)
+        rates.append(D * N / (time.perf_counter() - t0))
+    model, avail = cpu_model()
     ref = None
     exe = os.path.join(ROOT, "oracle", "_ref", "cpu_dedisp")
     if os.path.exists(exe):
         # the reference's own single-threaded brute force (src/prototypes/cpu_dedisp.c)
         try:
             nref = 16384
-            r = subprocess.run([exe, "-nchans", "1024", "-nsamp", str(nref), "-tdms", "64"],
-                               capture_output=True, text=True, timeout=120)
-            ms = float([ln for ln in r.stdout.splitlines() if ln.startswith("Time:")][0].split()[1])
-            ref = {"value": 64 * nref / (ms / 1e3), "unit": "DM*samples/s", "cores": 1,
-                   "kind": "reference", "sample": f"src/prototypes/cpu_dedisp.c, 1024 chans x 64 DMs x {nref} samples, time-major"}
+            rr = []
+            for _ in range(3):
+                r = subprocess.run([exe, "-nchans", "1024", "-nsamp", str(nref), "-tdms", "64"],
+                                   capture_output=True, text=True, timeout=120)
+                ms = float([ln for ln in r.stdout.splitlines() if ln.startswith("Time:")][0].split()[1])
+                rr.append(64 * nref / (ms / 1e3))
+            ref = {"value": float(np.median(rr)), "unit": "DM*samples/s", "cores": 1, "kind": "reference",
+                   "runs": rr, "cpu": model,
+                   "sample": f"src/prototypes/cpu_dedisp.c, 1024 chans x 64 DMs x {nref} samples, time-major, "
+                             "median of 3 runs"}
         except Exception as e:  # report, never fail the bench on the side measurement
             ref = {"error": str(e)}
-    cpu = {"value": D * N / el, "unit": "DM*samples/s", "cores": O.num_threads(), "kind": "port",
+    cpu = {"value": float(np.median(rates)), "unit": "DM*samples/s", "cores": O.num_threads(),
+           "kind": "port", "runs": rates, "cpu": model, "cpus_available": avail,
            "sample": f"1024 chans x {D} DMs (step 0.2) x {N} samples of the single-beam survey, "
-                     f"FP32 channel order, {el:.1f} s on {platform.processor() or platform.machine()}"}
+                     f"FP32 channel order, median of {runs} runs of about {run_s:.1f} s, "
+                     f"{O.num_threads()} OpenMP threads"}
     return cpu, ref
 
 
@@ -682,6 +711,7 @@ def main():
     value = total_work / ms16 * 1e3
     achieved = B16 / (ms16 / 1e3)
     traffic = None
+    traffic_source = None
     if os.path.exists(TRAFFIC_FILE):  # measured HBM bytes per launch (rocprofv3 --pmc)
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
@@ -689,6 +719,9 @@ def main():
         entry = tr.get("by_variant", {}).get(str(v16))
         if entry:
             traffic = entry.get("bytes_per_launch")
+            traffic_source = (f"stored profile {os.path.relpath(TRAFFIC_FILE, ROOT)}: rocprofv3 --pmc "
+                              f"FETCH_SIZE and WRITE_SIZE of this tile variant on this workload, "
+                              "calibrated by a device copy; not measured in this run")
         else:
             log(f"no traffic profile of tile variant {v16} (profiled: {sorted(tr.get('by_variant', {}))}): "
                 "roofline.traffic left null")
@@ -711,7 +744,7 @@ def main():
                    "beams": len(beams), "sum_dm_samples": total_work, "sum_adds": total_adds,
                    "tile_variant": v16, "launch_shape": modes16},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_source,
                      "algorithmic_bytes_per_launch": B16, "measured_copy_gbs": copy_gbs},
         "valu": {"adds_per_s": total_adds / (ms16 / 1e3), "peak_adds_per_s": VALU_ADD_PEAK,
                  "frac": total_adds / (ms16 / 1e3) / VALU_ADD_PEAK},

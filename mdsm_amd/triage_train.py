@@ -66,7 +66,8 @@ def build_stream(cfg, seed, max_events, max_chunks=100000, log=print):
     return d_snip, events[:n], stats, time.time() - t0
 
 
-THRESHOLDS = (0.9, 0.5, 0.3, 0.2, 0.1, 0.05, 0.02, 0.01, 0.005, 0.002, 0.001, 0.0005, 0.0002, 0.0001)
+THRESHOLDS = (0.9, 0.5, 0.3, 0.2, 0.1, 0.05, 0.02, 0.01, 0.005, 0.002, 0.001, 0.0005, 0.0002, 0.0001,
+              5e-5, 2e-5, 1e-5, 5e-6, 2e-6, 1e-6, 5e-7, 2e-7, 1e-7, 1e-8)
 
 
 def recall_fp(prob, events, stats, t):
@@ -103,6 +104,46 @@ def build_mixed(cfgs, seeds, max_events, log=print):
     return d_snip, events[:n], total, time.time() - t0
 
 
+def oversample_positives(d_snip, events, k, log=print):
+    """A training buffer in which every labelled-pulse event appears k times
+    (rows copied on the device) and every other event once: weights the
+    pulse class k:1 in the cross-entropy without touching the loss kernel.
+    Returns (buffer, labels, n)."""
+    pos = np.flatnonzero(events["label"] == 1)
+    n0 = len(events)
+    n = n0 + (k - 1) * len(pos)
+    d = abi.DeviceBuffer(n * ROW * 2)
+    row = ROW * 2
+    abi.check(abi.lib().mdsm_memcpy_dtod(d.ptr, d_snip.ptr, n0 * row, None))
+    labels = np.concatenate([events["label"].astype(np.int32), np.ones((k - 1) * len(pos), np.int32)])
+    j = n0
+    for _ in range(k - 1):
+        for i in pos.tolist():
+            abi.check(abi.lib().mdsm_memcpy_dtod(d.ptr + j * row, d_snip.ptr + i * row, row, None))
+            j += 1
+    abi.synchronize()
+    log(f"  oversampled {len(pos)} pulse events x{k}: {n} training rows")
+    return d, labels, n
+
+
+def cfg2_surveys(seed=2026):
+    """The beam shapes of the 32-beam uneven batch (nchans 512/1024, ndms
+    1024-4096, dmstep 0.1-0.5, 1500 MHz top over 300 MHz, 64 us), drawn as
+    the benchmark draws them, each searched in 2^17-sample chunks."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(32):
+        C = int(rng.choice([512, 1024]))
+        N = int(rng.integers(1 << 17, (1 << 20) + 1))  # (the batch's length; chunks are 2^17 here)
+        M = int(rng.integers(1024, 4097))
+        step = float(rng.uniform(0.1, 0.5))
+        for _ in range(20):  # (the batch's pulse draws, to keep the generator in step)
+            rng.uniform(10.0, 800.0), rng.uniform(0, np.log(64)), rng.uniform(8, 30), rng.integers(0, N - 64)
+        out.append(dict(fch1=1500.0, foff=-300.0 / C, nchans=C, tsamp=64e-6, lowdm=0.0, dmstep=step,
+                        ndms=M, nsamp=1 << 17))
+    return out
+
+
 def score(prob, events, stats, thresholds):
     """Plain vs learned triage on one held-out stream."""
     out = {"plain": {"recall": stats["injected_found"] / stats["injected"], "false_positives":
@@ -125,6 +166,10 @@ def main(argv=None):
                     help="operating threshold (default: picked on a validation stream)")
     ap.add_argument("--single-survey", action="store_true",
                     help="train on the single-beam survey only (default: three beam shapes)")
+    ap.add_argument("--pos-weight", type=int, default=3,
+                    help="pulse events repeated this many times in the training set (class weight)")
+    ap.add_argument("--cfg2-chunks", type=int, default=4,
+                    help="chunks per beam of the held-out stream over the 32 batch beam shapes (0: skip)")
     ap.add_argument("--save", default=None, help="write trained parameters (.npy, canonical layout)")
     ap.add_argument("--json", default=None, help="write the results as JSON")
     args = ap.parse_args(argv)
@@ -143,18 +188,25 @@ def main(argv=None):
                                                 args.events, log=log)
     res["train"] = {"events": len(ev_tr), "pulses": int(ev_tr["label"].sum()), "stats": st_tr,
                     "sim_seconds": t_sim}
-    lab = abi.DeviceBuffer.from_numpy(ev_tr["label"].astype(np.int32))
+    n_tr = len(ev_tr)
+    if args.pos_weight > 1:
+        d_tr, labels, n_tr = oversample_positives(d_tr, ev_tr, args.pos_weight, log=log)
+    else:
+        labels = ev_tr["label"].astype(np.int32)
+    res["train"]["rows"] = n_tr
+    res["train"]["pos_weight"] = args.pos_weight
+    lab = abi.DeviceBuffer.from_numpy(labels)
     net = abi.TriageNet(SNIP[0], SNIP[1], args.batch)
     net.init(args.seed)
     res["epochs"] = []
     for ep in range(args.epochs):
         t = time.time()
         lr = args.lr * (0.5 ** (ep // 2))
-        loss = abi.train_epoch(net, d_tr.ptr, lab.ptr, len(ev_tr), args.batch, lr, args.seed * 1000 + ep)
+        loss = abi.train_epoch(net, d_tr.ptr, lab.ptr, n_tr, args.batch, lr, args.seed * 1000 + ep)
         abi.synchronize()
         dt = time.time() - t
-        res["epochs"].append({"loss": loss, "seconds": dt, "events_per_s": len(ev_tr) / dt})
-        log(f"epoch {ep}: loss {loss:.4f}, {len(ev_tr) / dt:.0f} events/s")
+        res["epochs"].append({"loss": loss, "seconds": dt, "events_per_s": n_tr / dt})
+        log(f"epoch {ep}: loss {loss:.4f}, {n_tr / dt:.0f} events/s")
     del d_tr
     # operating point from a validation stream of the single-beam survey
     d_va, ev_va, st_va, _ = build_stream(make_config(TRAIN_SURVEY), args.seed + 50, 2000 * args.eval_chunks,
@@ -199,6 +251,47 @@ def main(argv=None):
         r["inference_events_per_s"] = n / max(t_inf, 1e-9)
         res["eval"].append(r)
         log(json.dumps(r))
+    if args.cfg2_chunks > 0:
+        # held-out streams over the 32 beam shapes of the batch, aggregated
+        # (configs: inference on the batch's streams, the plain path judging)
+        agg = {"plain_found": 0, "injected": 0, "plain_fp": 0, "clusters": 0, "events": 0,
+               "cnn_found": {}, "cnn_fp": {}}
+        t_all = time.time()
+        for b, sv in enumerate(cfg2_surveys()):
+            cap = 2000 * args.cfg2_chunks
+            d_te, ev_te, st_te, _ = build_stream(make_config(sv), args.seed + 1000 + b, cap,
+                                                 max_chunks=args.cfg2_chunks, log=lambda m: None)
+            if len(ev_te) >= cap:
+                raise RuntimeError("batch-shape stream filled its event buffer; raise the capacity")
+            n = len(ev_te)
+            d_prob = abi.DeviceBuffer(4 * max(n, 1))
+            abi.predict(net, d_te.ptr, n, d_prob.ptr)
+            abi.synchronize()
+            prob = d_prob.download(np.float32, (n,))
+            agg["plain_found"] += st_te["injected_found"]
+            agg["injected"] += st_te["injected"]
+            agg["plain_fp"] += st_te["false_clusters"]
+            agg["clusters"] += st_te["clusters"]
+            agg["events"] += n
+            for t in sorted(set((thr, thr_half) + THRESHOLDS)):
+                keep = prob > t
+                hits = keep & (ev_te["label"] == 1)
+                found = len(set(zip(ev_te["chunk"][hits].tolist(), ev_te["pulse"][hits].tolist())))
+                agg["cnn_found"][t] = agg["cnn_found"].get(t, 0) + found
+                agg["cnn_fp"][t] = agg["cnn_fp"].get(t, 0) + int(((ev_te["label"] == 0) & keep).sum())
+            del d_te, d_prob
+        c2 = {"beams": 32, "chunks_per_beam": args.cfg2_chunks, "seconds": time.time() - t_all,
+              "plain": {"recall": agg["plain_found"] / max(agg["injected"], 1),
+                        "false_positives": agg["plain_fp"], "clusters": agg["clusters"]},
+              "injected": agg["injected"]}
+        for t in sorted(set((thr, thr_half) + THRESHOLDS)):
+            c2[f"cnn@{t}"] = {"recall": agg["cnn_found"][t] / max(agg["injected"], 1),
+                              "false_positives": agg["cnn_fp"][t]}
+        c2["operating_point"] = c2[f"cnn@{thr}"]
+        c2["operating_point_half"] = c2[f"cnn@{thr_half}"]
+        res["eval_batch_shapes"] = c2
+        log(json.dumps({"batch_shapes": {k: c2[k] for k in ("plain", "operating_point",
+                                                           "operating_point_half", "injected")}}))
     if args.save:
         np.save(args.save, net.params())
     if args.json:
