@@ -305,18 +305,16 @@ __global__ __launch_bounds__(NT) void detect_wave_kernel(
     const double den = sd * kSqrtW[k];
     const bool in_range = den > 0x1p-960 && den < 0x1p960;
     const double bound = in_range ? lim * den : -INFINITY;
-    double a[DW_R];
     bool any = false;
 #pragma unroll
-    for (int i = 0; i < DW_R; ++i) {
-      a[i] = v[i] - wm;
-      any |= a[i] >= bound;
-    }
+    for (int i = 0; i < DW_R; ++i) any |= v[i] - wm >= bound;
     if (live && any) {
+      // (a is formed again here, so the common path holds no array of it)
 #pragma unroll
       for (int i = 0; i < DW_R; ++i) {
-        if ((int64_t)i + (int64_t)(1u << k) <= room && a[i] >= bound) {
-          const double snr = a[i] / den;
+        const double a = v[i] - wm;
+        if ((int64_t)i + (int64_t)(1u << k) <= room && a >= bound) {
+          const double snr = a / den;
           if (snr > best[i]) {
             best[i] = snr;
             if (i < 8) bk = (bk & ~(15u << (4 * i))) | ((uint32_t)k << (4 * i));
@@ -331,20 +329,19 @@ __global__ __launch_bounds__(NT) void detect_wave_kernel(
   for (int k = 1; k < 7; ++k) {
     if (k >= nwidths) break;
     const int h = 1 << (k - 1);
-    double nx[DW_R];
     if (h < DW_R) {
-      // the first h values of the next lane
+      // the first h values of the next lane, taken before any is updated
+      double nx[DW_R / 2];
 #pragma unroll
-      for (int j = 0; j < DW_R; ++j)
+      for (int j = 0; j < DW_R / 2; ++j)
         if (j < h) nx[j] = __shfl_down(v[j], 1);
 #pragma unroll
       for (int i = 0; i < DW_R; ++i) v[i] = v[i] + (i + h < DW_R ? v[i + h] : nx[i + h - DW_R]);
     } else {
-      // h = 16: all of the next lane; h = 32: all of the lane after it
+      // h = 16: the same element of the next lane; h = 32: of the lane after
+      // it (every lane reads the old value before adding)
 #pragma unroll
-      for (int j = 0; j < DW_R; ++j) nx[j] = __shfl_down(v[j], h / DW_R);
-#pragma unroll
-      for (int i = 0; i < DW_R; ++i) v[i] = v[i] + nx[i];
+      for (int i = 0; i < DW_R; ++i) v[i] = v[i] + __shfl_down(v[i], h / DW_R);
     }
     test_level(k);
   }
