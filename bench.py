@@ -395,6 +395,36 @@ def compat_throughput(nsamp=1 << 18, ndms=4096, nchunks=10, storage="fp32"):
             "pcie_gbs": (Cn * N * 4 + M * N * 4) / per / 1e9, "chunks_timed": nchunks - 2}
 
 
+def detect_run(steps, rows=2048, nsamp=1 << 20):
+    """The boxcar detector alone on a rows x nsamp FP32 array of device
+    noise: block statistics (L = 16384) and detection (7 widths, 6 sigma)
+    with the register-resident detector, then with a block length that is
+    not a multiple of 16 (L = 16383, the LDS-tiled detector), and a device
+    copy of the same bytes for the attainable rate.  Reports milliseconds
+    and read rates (the workload of the detector's counter profiles)."""
+    s = abi.Survey.make(FCH1, -BAND / rows, rows, TSAMP, 0.0, 0.2, 16, nsamp)
+    B = rows * nsamp * 4
+    d = abi.DeviceBuffer(B)
+    abi.generate(3, s, 0, nsamp, d.ptr, nsamp, np.ones(rows, np.float32), np.zeros(rows, np.float32), [])
+    cap = 1 << 22
+    dc, dn = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+    out = {"rows": rows, "nsamp": nsamp, "bytes": B}
+    for L, name in ((16384, "register_resident"), (16383, "lds_tiled")):
+        nb = (nsamp + L - 1) // L
+        dm, ds = abi.DeviceBuffer(rows * nb * 8), abi.DeviceBuffer(rows * nb * 8)
+        ms_bs = time_steps(lambda: abi.block_stats(d.ptr, rows, nsamp, L, dm.ptr, ds.ptr), steps, 1, None)
+        ms = time_steps(lambda: abi.detect(d.ptr, rows, nsamp, L, dm.ptr, ds.ptr, 7, 6.0, dc.ptr, cap, dn.ptr),
+                        steps, 1, None)
+        out[name] = {"block": L, "detect_ms": ms, "detect_read_gbs": B / ms / 1e6, "block_stats_ms": ms_bs,
+                     "block_stats_read_gbs": B / ms_bs / 1e6,
+                     "candidates": int(dn.download(np.uint32, (1,))[0])}
+    dst = abi.DeviceBuffer(B)
+    ms = time_steps(lambda: abi.check(abi.lib().mdsm_memcpy_dtod(dst.ptr, d.ptr, B, None)), steps, 1, None)
+    out["copy_ms"] = ms
+    out["copy_gbs_read_plus_write"] = 2 * B / ms / 1e6
+    return out
+
+
 def tile_run(spec, steps):
     """One dedispersion tile shape on the single-beam survey truncated to
     2^18 samples (1024 channels x 4096 trials of 0.2), `steps` launches: the
@@ -436,6 +466,9 @@ def main():
     ap.add_argument("--tile", default=None,
                     help="only time one tile shape ('<f32|f16|u8>:<variant>') at 1024 ch x 2^18 x 4096, "
                          "the workload of the per-tile counter profiles")
+    ap.add_argument("--detect", action="store_true",
+                    help="only time the boxcar detector on a 2048 x 2^20 array (both detector forms) "
+                         "beside a device copy")
     ap.add_argument("--compat", action="store_true",
                     help="only measure the drop-in manager API with host buffers (PCIe included)")
     ap.add_argument("--no-pipeline", dest="pipeline", action="store_false",
@@ -452,6 +485,11 @@ def main():
         res = tile_run(args.tile, args.steps)
         log(f"[tile] {res}")
         print(json.dumps({"tile": res}))
+        return
+    if args.detect:
+        res = detect_run(args.steps)
+        log(f"[detect] {res}")
+        print(json.dumps({"detect": res}))
         return
     if args.compat:
         res = {st: compat_throughput(storage=st) for st in ("fp32", "fp16")}

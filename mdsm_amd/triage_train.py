@@ -78,6 +78,44 @@ def recall_fp(prob, events, stats, t):
     return found / max(stats["injected"], 1), int(((events["label"] == 0) & keep).sum())
 
 
+def recall_fp_rule(prob, events, stats, t, snr_keep):
+    """As recall_fp, for the rule that keeps an event when the network's
+    probability exceeds t or the plain path's S/N reaches snr_keep."""
+    keep = (prob > t) | (events["snr"] >= snr_keep)
+    hits = keep & (events["label"] == 1)
+    found = len(set(zip(events["chunk"][hits].tolist(), events["pulse"][hits].tolist())))
+    return found / max(stats["injected"], 1), int(((events["label"] == 0) & keep).sum())
+
+
+SNR_KEEP = (1e9, 40.0, 30.0, 25.0, 20.0, 16.0, 14.0, 12.0, 10.0, 9.0, 8.0)
+
+
+def hybrid_rule(prob, events, stats, t, fp_fraction=0.5):
+    """The lowest S/N rescue level whose false positives, together with the
+    network's at threshold t, stay under fp_fraction of the plain path's on
+    a validation stream (1e9: no rescue)."""
+    budget = fp_fraction * stats["false_clusters"]
+    ok = [x for x in SNR_KEEP if recall_fp_rule(prob, events, stats, t, x)[1] <= budget]
+    return min(ok) if ok else SNR_KEEP[0]
+
+
+def missed_pulses(prob, events, t):
+    """Features of the best event (highest S/N) of every injected pulse that
+    the plain path found but no event of which the network keeps at t."""
+    lab = events["label"] == 1
+    keys = set(zip(events["chunk"][lab].tolist(), events["pulse"][lab].tolist()))
+    kept = set(zip(events["chunk"][lab & (prob > t)].tolist(), events["pulse"][lab & (prob > t)].tolist()))
+    out = []
+    for ch, pu in sorted(keys - kept):
+        sel = np.flatnonzero(lab & (events["chunk"] == ch) & (events["pulse"] == pu))
+        b = sel[np.argmax(events["snr"][sel])]
+        out.append({"chunk": ch, "pulse": pu, "snr": float(events["snr"][b]),
+                    "width_log2": int(events["width_log2"][b]), "dm_idx": int(events["dm_idx"][b]),
+                    "sample": int(events["sample"][b]), "cluster_size": int(events["cluster_size"][b]),
+                    "prob": float(prob[b]), "events": int(len(sel))})
+    return out
+
+
 def operating_threshold(prob, events, stats, fp_fraction=0.25):
     """Lowest threshold whose false positives stay under fp_fraction of the
     plain path's on a validation stream (lower thresholds keep more pulses)."""
@@ -104,25 +142,25 @@ def build_mixed(cfgs, seeds, max_events, log=print):
     return d_snip, events[:n], total, time.time() - t0
 
 
-def oversample_positives(d_snip, events, k, log=print):
+def oversample_positives(d_snip, events, k, k_wide=1, log=print):
     """A training buffer in which every labelled-pulse event appears k times
-    (rows copied on the device) and every other event once: weights the
-    pulse class k:1 in the cross-entropy without touching the loss kernel.
-    Returns (buffer, labels, n)."""
-    pos = np.flatnonzero(events["label"] == 1)
+    (k * k_wide times for the widest boxcars, 2^5 and 2^6 samples, whose
+    weak members the network otherwise rates as RFI), rows copied on the
+    device, and every other event once: weights the pulse class without
+    touching the loss kernel.  Returns (buffer, labels, n)."""
+    lab = events["label"] == 1
+    reps = np.where(lab, k * np.where(events["width_log2"] >= 5, k_wide, 1), 1).astype(np.int64)
+    extra = np.repeat(np.arange(len(events)), reps - 1)
     n0 = len(events)
-    n = n0 + (k - 1) * len(pos)
+    n = n0 + len(extra)
     d = abi.DeviceBuffer(n * ROW * 2)
     row = ROW * 2
     abi.check(abi.lib().mdsm_memcpy_dtod(d.ptr, d_snip.ptr, n0 * row, None))
-    labels = np.concatenate([events["label"].astype(np.int32), np.ones((k - 1) * len(pos), np.int32)])
-    j = n0
-    for _ in range(k - 1):
-        for i in pos.tolist():
-            abi.check(abi.lib().mdsm_memcpy_dtod(d.ptr + j * row, d_snip.ptr + i * row, row, None))
-            j += 1
+    for j, i in enumerate(extra.tolist()):
+        abi.check(abi.lib().mdsm_memcpy_dtod(d.ptr + (n0 + j) * row, d_snip.ptr + i * row, row, None))
     abi.synchronize()
-    log(f"  oversampled {len(pos)} pulse events x{k}: {n} training rows")
+    labels = np.concatenate([events["label"].astype(np.int32), events["label"][extra].astype(np.int32)])
+    log(f"  oversampled {int(lab.sum())} pulse events (x{k}, widest boxcars x{k * k_wide}): {n} training rows")
     return d, labels, n
 
 
@@ -166,8 +204,10 @@ def main(argv=None):
                     help="operating threshold (default: picked on a validation stream)")
     ap.add_argument("--single-survey", action="store_true",
                     help="train on the single-beam survey only (default: three beam shapes)")
-    ap.add_argument("--pos-weight", type=int, default=3,
+    ap.add_argument("--pos-weight", type=int, default=1,
                     help="pulse events repeated this many times in the training set (class weight)")
+    ap.add_argument("--wide-weight", type=int, default=1,
+                    help="extra repetition of pulse events found at the two widest boxcars")
     ap.add_argument("--cfg2-chunks", type=int, default=4,
                     help="chunks per beam of the held-out stream over the 32 batch beam shapes (0: skip)")
     ap.add_argument("--save", default=None, help="write trained parameters (.npy, canonical layout)")
@@ -190,11 +230,12 @@ def main(argv=None):
                     "sim_seconds": t_sim}
     n_tr = len(ev_tr)
     if args.pos_weight > 1:
-        d_tr, labels, n_tr = oversample_positives(d_tr, ev_tr, args.pos_weight, log=log)
+        d_tr, labels, n_tr = oversample_positives(d_tr, ev_tr, args.pos_weight, args.wide_weight, log=log)
     else:
         labels = ev_tr["label"].astype(np.int32)
     res["train"]["rows"] = n_tr
     res["train"]["pos_weight"] = args.pos_weight
+    res["train"]["wide_weight"] = args.wide_weight
     lab = abi.DeviceBuffer.from_numpy(labels)
     net = abi.TriageNet(SNIP[0], SNIP[1], args.batch)
     net.init(args.seed)
@@ -224,6 +265,11 @@ def main(argv=None):
     res["validation"] = score(pv, ev_va, st_va, THRESHOLDS)
     res["threshold"] = thr
     res["threshold_half"] = thr_half
+    # the hybrid rule at the half budget: the network at thr_half, or the
+    # plain path's S/N at the lowest rescue level the budget allows
+    snr_keep = hybrid_rule(pv, ev_va, st_va, thr_half)
+    res["hybrid_snr_keep"] = snr_keep
+    log(f"hybrid rule: prob > {thr_half} or S/N >= {snr_keep}")
     log(f"operating thresholds {thr} (quarter of the plain false positives), {thr_half} (half)")
     del d_va, d_prob
     thresholds = THRESHOLDS
@@ -248,14 +294,18 @@ def main(argv=None):
         r["survey"] = sv
         r["operating_point"] = r[f"cnn@{thr}"]
         r["operating_point_half"] = r[f"cnn@{thr_half}"]
+        hr, hf = recall_fp_rule(prob, ev_te, st_te, thr_half, snr_keep)
+        r["hybrid_half"] = {"recall": hr, "false_positives": hf, "snr_keep": snr_keep}
+        r["missed_at_half"] = missed_pulses(prob, ev_te, thr_half)
         r["inference_events_per_s"] = n / max(t_inf, 1e-9)
         res["eval"].append(r)
-        log(json.dumps(r))
+        log(json.dumps({k: r[k] for k in ("plain", "operating_point", "operating_point_half", "hybrid_half",
+                                          "missed_at_half")}))
     if args.cfg2_chunks > 0:
         # held-out streams over the 32 beam shapes of the batch, aggregated
         # (configs: inference on the batch's streams, the plain path judging)
         agg = {"plain_found": 0, "injected": 0, "plain_fp": 0, "clusters": 0, "events": 0,
-               "cnn_found": {}, "cnn_fp": {}}
+               "cnn_found": {}, "cnn_fp": {}, "hyb_found": 0, "hyb_fp": 0}
         t_all = time.time()
         for b, sv in enumerate(cfg2_surveys()):
             cap = 2000 * args.cfg2_chunks
@@ -273,6 +323,9 @@ def main(argv=None):
             agg["plain_fp"] += st_te["false_clusters"]
             agg["clusters"] += st_te["clusters"]
             agg["events"] += n
+            hr, hf = recall_fp_rule(prob, ev_te, st_te, thr_half, snr_keep)
+            agg["hyb_found"] += round(hr * st_te["injected"])
+            agg["hyb_fp"] += hf
             for t in sorted(set((thr, thr_half) + THRESHOLDS)):
                 keep = prob > t
                 hits = keep & (ev_te["label"] == 1)
@@ -289,9 +342,12 @@ def main(argv=None):
                               "false_positives": agg["cnn_fp"][t]}
         c2["operating_point"] = c2[f"cnn@{thr}"]
         c2["operating_point_half"] = c2[f"cnn@{thr_half}"]
+        c2["hybrid_half"] = {"recall": agg["hyb_found"] / max(agg["injected"], 1),
+                             "false_positives": agg["hyb_fp"], "snr_keep": snr_keep}
         res["eval_batch_shapes"] = c2
         log(json.dumps({"batch_shapes": {k: c2[k] for k in ("plain", "operating_point",
-                                                           "operating_point_half", "injected")}}))
+                                                           "operating_point_half", "hybrid_half",
+                                                           "injected")}}))
     if args.save:
         np.save(args.save, net.params())
     if args.json:
