@@ -138,13 +138,15 @@ def spot_check(beam, plan_like_table, d_in, ld, dtype, d_out, rows, want_error=F
     """Bit-exact check of selected full-length DM rows against the oracle."""
     import oracle as O
     s = beam["survey"]
-    np_dt = {abi.F32: np.float32, abi.F16: np.uint16, abi.U8: np.uint8}[dtype]
+    np_dt = {abi.F32: np.float32, abi.F16: np.uint16, abi.U8: np.uint8, abi.F16_PAIRS: np.uint16}[dtype]
     x = d_in.download(np_dt, (s.nchans, ld)) if hasattr(d_in, "download") else None
     t = plan_like_table[:, rows]
     if dtype == abi.F32:
         ref = O.dedisperse_f32seq(x, np.ascontiguousarray(t), s.nsamp)
     elif dtype == abi.F16:
         ref = O.dedisperse_f16(x, np.ascontiguousarray(t), s.nsamp)
+    elif dtype == abi.F16_PAIRS:
+        ref = O.dedisperse_f16_pairs(x, np.ascontiguousarray(t), s.nsamp)
     else:
         ref = O.dedisperse_u8(x, np.ascontiguousarray(t), s.nsamp)
     ok = True
@@ -267,7 +269,7 @@ def precision_report(beams, prec_rows, L=16384, thr=6.0, nwidths=7):
     rows).  The raw input is regenerated on the host by the oracle's
     generator, which the device generator matches bit for bit."""
     import oracle as O
-    conv = {abi.F32: "f32", abi.F16: "f16", abi.U8: "u8"}
+    conv = {abi.F32: "f32", abi.F16: "f16", abi.U8: "u8", abi.F16_PAIRS: "f16_pairs"}
     out = []
     for b in sorted({b for _, b in prec_rows}):
         beam = beams[b]
@@ -292,6 +294,8 @@ def precision_report(beams, prec_rows, L=16384, thr=6.0, nwidths=7):
                 ref = O.dedisperse_f32seq(x32, tab, s.nsamp)
             elif dt == abi.F16:
                 ref = O.dedisperse_f16(O.normalise(raw, m, sd, "f16", ld), tab, s.nsamp)
+            elif dt == abi.F16_PAIRS:
+                ref = O.dedisperse_f16_pairs(O.normalise(raw, m, sd, "f16", ld), tab, s.nsamp)
             else:
                 ref = O.dedisperse_u8(O.normalise(raw, m, sd, "u8", ld), tab, s.nsamp)
             err = got.astype(np.float64) - y64
@@ -429,12 +433,12 @@ def tile_run(spec, steps):
     """One dedispersion tile shape on the single-beam survey truncated to
     2^18 samples (1024 channels x 4096 trials of 0.2), `steps` launches: the
     workload the per-tile rocprofv3 counter passes profile.  spec is
-    '<f32|f16|u8>:<variant>'."""
+    '<f32|f16|f16p|u8>:<variant>'."""
     name, v = spec.split(":")
-    dt = {"f32": abi.F32, "f16": abi.F16, "u8": abi.U8}[name]
+    dt = {"f32": abi.F32, "f16": abi.F16, "u8": abi.U8, "f16p": abi.F16_PAIRS}[name]
     s = abi.Survey.make(FCH1, -BAND / 1024, 1024, TSAMP, 0.0, 0.2, 4096, 1 << 18)
     plan = abi.Plan(s, dt)
-    isz = {abi.F32: 4, abi.F16: 2, abi.U8: 1}[dt]
+    isz = {abi.F32: 4, abi.F16: 2, abi.U8: 1, abi.F16_PAIRS: 2}[dt]
     din = abi.DeviceBuffer(1024 * plan.min_ld * isz)
     din.zero()
     dout = abi.DeviceBuffer(4096 * (1 << 18) * 4)
@@ -464,7 +468,7 @@ def main():
                     help="tile variant of the batched FP16 launch instead of the autotuner's "
                          "(for profiling one tile shape)")
     ap.add_argument("--tile", default=None,
-                    help="only time one tile shape ('<f32|f16|u8>:<variant>') at 1024 ch x 2^18 x 4096, "
+                    help="only time one tile shape ('<f32|f16|f16p|u8>:<variant>') at 1024 ch x 2^18 x 4096, "
                          "the workload of the per-tile counter profiles")
     ap.add_argument("--detect", action="store_true",
                     help="only time the boxcar detector on a 2048 x 2^20 array (both detector forms) "
@@ -560,7 +564,7 @@ def main():
 
     def run_batch(dtype, tag, ins=None, v=None):
         batch = abi.Batch(surveys, dtype)
-        isz = {abi.F32: 4, abi.F16: 2, abi.U8: 1}[dtype]
+        isz = {abi.F32: 4, abi.F16: 2, abi.U8: 1, abi.F16_PAIRS: 2}[dtype]
         if ins is None:
             st = Stage(stream)
             ins = []
@@ -638,6 +642,23 @@ def main():
         for d, _ in ins8:
             d.free()
         del batch8
+
+    # FP16 storage with channel-pair sums (one FP16 rounding per pair before
+    # the FP32 accumulation): the same inputs as the FP16 run, regenerated
+    pairs = None
+    if not args.profile:
+        batchp, insp, msp, Bp, vp = run_batch(abi.F16_PAIRS, "fp16 pairs")
+        okp = spot_check(beams[0], t0tab, insp[0][0], insp[0][1], abi.F16_PAIRS, outs[0],
+                         [0, surveys[0].ndms // 2, surveys[0].ndms - 1])
+        pairs = {"ms": msp, "dm_samples_per_s": total_work / msp * 1e3, "tile_variant": vp,
+                 "launch_shape": batchp.mode_ms, "hbm_frac": Bp / (msp / 1e3) / HBM_PEAK,
+                 "adds_per_s": total_adds / (msp / 1e3), "spot_check_bitexact": okp,
+                 "arithmetic": "channels 2k, 2k+1 added in FP16 (one IEEE rounding), "
+                               "pair sums accumulated in FP32 in channel order"}
+        log(f"[batch fp16 pairs] spot check {okp}")
+        for d, _ in insp:
+            d.free()
+        del batchp
 
     plain = None
     if not args.no_plain:
@@ -791,6 +812,9 @@ def main():
         "lds": {"achieved": 2.0 * total_adds / (ms16 / 1e3), "peak": LDS_PEAK, "unit": "B/s",
                 "frac": 2.0 * total_adds / (ms16 / 1e3) / LDS_PEAK, "bytes_per_add": 2},
         "batched_u8": u8,
+        "batched_fp16_pairs": pairs,
+        "batched_fp16_pairs_vs_plain_fp32": (pairs["dm_samples_per_s"] / plain["dm_samples_per_s"])
+        if (pairs and plain) else None,
         "batched_u8_vs_plain_fp32": (u8["dm_samples_per_s"] / plain["dm_samples_per_s"]) if (u8 and plain) else None,
         "plain_fp32_per_beam": plain,
         "batched_fp16_vs_plain_fp32": (value / plain["dm_samples_per_s"]) if plain else None,

@@ -38,8 +38,14 @@ enum {
   MDSM_ERR_FORMAT = 7    /* malformed SIGPROC header */
 };
 
-/* Input sample storage types. */
-enum { MDSM_F32 = 0, MDSM_F16 = 1, MDSM_U8 = 2 };
+/* Input sample storage types.  MDSM_F16_PAIRS stores halves exactly as
+ * MDSM_F16 (the same rows, produced by mdsm_normalise with either code) and
+ * changes only the dedisperser's arithmetic: channels 2k and 2k+1 of a trial
+ * are added in FP16 (one IEEE round-to-nearest-even) and that pair sum is
+ * added to the FP32 channel-order accumulator (an odd last channel alone),
+ * which halves the FP32 adds; its rounding error against FP64 is about 1.7x
+ * that of MDSM_F16. */
+enum { MDSM_F32 = 0, MDSM_F16 = 1, MDSM_U8 = 2, MDSM_F16_PAIRS = 3 };
 
 /* Delay-table conventions.
  * FP64_ROUND: D[c][d] = round(4148.808*DM_d*(f_c^-2 - f_top^-2)/tsamp), FP64.
@@ -148,8 +154,10 @@ int mdsm_plan_autotune(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_o
  * brute_force_dedispersion (src/mdsm/dedispersion_thread.cu:220-243).
  * d_in dtype is the plan's: F32 floats, F16 halves, or U8 codes whose value
  * is u*(8/255) - 4 in normalised units (then the sum is exact in integers and
- * out = (float)(S*(8/255) - 4*nchans)).  Rows must start on 4-byte
- * boundaries: ld a multiple of 2 (F16) or 4 (U8), rows shorter than 2 GiB;
+ * out = (float)(S*(8/255) - 4*nchans)); F16_PAIRS halves are summed as
+ * out = sum_k fl32(fl16(in[2k] + in[2k+1])) in channel-pair order.  Rows must
+ * start on 4-byte boundaries: ld a multiple of 2 (F16) or 4 (U8), rows
+ * shorter than 2 GiB;
  * mdsm_plan_info reports the smallest valid ld.  A plan (like the
  * reference's per-beam thread state) is used by one host thread at a time:
  * the first call with a tile variant builds that variant's offset table. */
@@ -157,7 +165,8 @@ int mdsm_dedisperse(mdsm_plan plan, const void *d_in, uint64_t ld, float *d_out,
 
 /* --------------------------------------------------- bandpass normalisation
  * Per-channel FP64 mean/std over n samples of d_raw[nchans][ld] (fixed
- * reduction order), then v = (x - mean) * (1/std) stored as F32, F16 or U8
+ * reduction order), then v = (x - mean) * (1/std) stored as F32, F16 (or
+ * F16_PAIRS, the same halves) or U8
  * (U8: clip to [-4,4], u = floor((v+4)*255/8 + 0.5)).  The per-channel
  * statistics of the reference bandpass stage (bandpass_power_sum,
  * multibeam_dedispersion_kernel.cu:317-350) feed a normaliser here. */

@@ -96,7 +96,8 @@ typedef struct {
   unsigned num_gpus;
   /* Extension, after the reference's fields so that their offsets are
    * unchanged: input storage of the dedisperser, MDSM_F32 (0, the
-   * reference's), MDSM_F16 (1) or MDSM_U8 (2), from <storage type="..."/>. */
+   * reference's), MDSM_F16 (1), MDSM_U8 (2) or MDSM_F16_PAIRS (3), from
+   * <storage type="fp32|fp16|u8|fp16pairs"/>. */
   unsigned input_storage;
 } SURVEY;
 

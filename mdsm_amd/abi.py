@@ -17,11 +17,12 @@ LIB_PATH = os.path.join(_HERE, "libmdsm_amd.so")
 
 (MDSM_OK, MDSM_ERR_ARG, MDSM_ERR_HIP, MDSM_ERR_NOMEM, MDSM_ERR_OVERFLOW, MDSM_ERR_STATE,
  MDSM_ERR_IO, MDSM_ERR_FORMAT) = range(8)
-F32, F16, U8 = 0, 1, 2
+# F16_PAIRS: F16 rows; channels 2k, 2k+1 added in FP16 before the FP32 sum
+F32, F16, U8, F16_PAIRS = 0, 1, 2, 3
 DELAY_FP64_ROUND, DELAY_REFERENCE, DELAY_LEGACY = 0, 1, 2
-_ITEMSIZE = {F32: 4, F16: 2, U8: 1}
-_NP = {F32: np.float32, F16: np.uint16, U8: np.uint8}
-ALIGN = {F32: 1, F16: 2, U8: 4}  # required multiple of the input row length
+_ITEMSIZE = {F32: 4, F16: 2, U8: 1, F16_PAIRS: 2}
+_NP = {F32: np.float32, F16: np.uint16, U8: np.uint8, F16_PAIRS: np.uint16}
+ALIGN = {F32: 1, F16: 2, U8: 4, F16_PAIRS: 2}  # required multiple of the input row length
 
 CAND_DTYPE = np.dtype([("dm_idx", "<i4"), ("sample", "<i4"), ("width_log2", "<i4"), ("snr", "<f4")])
 

@@ -302,7 +302,7 @@ static int plan_variant(const mdsm_plan_s *p) { return p->variant ? p->variant :
 extern "C" int mdsm_plan_create(const mdsm_survey_t *s, int in_dtype, mdsm_plan *plan) {
   MDSM_REQUIRE(plan, MDSM_ERR_ARG, "null plan");
   *plan = nullptr;
-  MDSM_REQUIRE(in_dtype == MDSM_F32 || in_dtype == MDSM_F16 || in_dtype == MDSM_U8, MDSM_ERR_ARG,
+  MDSM_REQUIRE(known_dtype(in_dtype), MDSM_ERR_ARG,
                "unknown input dtype %d", in_dtype);
   MDSM_REQUIRE(s && s->nchans <= 65536, MDSM_ERR_ARG, "nchans must be <= 65536");
   auto *p = new mdsm_plan_s();
@@ -387,7 +387,7 @@ extern "C" int mdsm_plan_set_variant(mdsm_plan plan, int variant) {
 
 extern "C" int mdsm_tile_preferred(int in_dtype, int *variant) {
   MDSM_REQUIRE(variant, MDSM_ERR_ARG, "null variant");
-  MDSM_REQUIRE(in_dtype == MDSM_F32 || in_dtype == MDSM_F16 || in_dtype == MDSM_U8, MDSM_ERR_ARG,
+  MDSM_REQUIRE(known_dtype(in_dtype), MDSM_ERR_ARG,
                "unknown input dtype %d", in_dtype);
   *variant = preferred_variant(in_dtype);
   return MDSM_OK;

@@ -79,7 +79,7 @@ static void free_batch(mdsm_batch_s *b) {
 extern "C" int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, int in_dtype,
                                  mdsm_batch *batch) {
   MDSM_REQUIRE(batch && beams && nbeams > 0, MDSM_ERR_ARG, "bad arguments");
-  MDSM_REQUIRE(in_dtype == MDSM_F32 || in_dtype == MDSM_F16 || in_dtype == MDSM_U8, MDSM_ERR_ARG,
+  MDSM_REQUIRE(known_dtype(in_dtype), MDSM_ERR_ARG,
                "unknown input dtype %d", in_dtype);
   for (uint32_t i = 0; i < nbeams; ++i)
     MDSM_REQUIRE(beams[i].nchans <= 65536, MDSM_ERR_ARG, "beam %u: nchans must be <= 65536", i);

@@ -214,11 +214,9 @@ void write_candidates(unsigned b, const ChunkInfo &ck, const std::vector<mdsm_ca
 
 float *out_ptr(int parity, unsigned b) { return g.d_out[parity] + (size_t)b * g.tdms * g.nsamp; }
 
-size_t storage_bytes(int dtype) { return dtype == MDSM_F32 ? 4 : (dtype == MDSM_F16 ? 2 : 1); }
-
 void *store_ptr(int parity, unsigned b) {
   return static_cast<char *>(g.d_store[parity]) +
-         (size_t)b * g.nchans * g.ld * storage_bytes(g.storage);
+         (size_t)b * g.nchans * g.ld * mdsm::storage_bytes(g.storage);
 }
 
 float *raw_ptr(int parity, unsigned b) { return g.d_raw[parity] + (size_t)b * g.nchans * g.ld; }
@@ -403,7 +401,7 @@ void initialiseMDSM(SURVEY *survey) {
     return;
   }
   const unsigned st = survey->input_storage;
-  if (st != MDSM_F32 && st != MDSM_F16 && st != MDSM_U8) {
+  if (!mdsm::known_dtype((int)st)) {
     fail("unknown input storage %u", st);
     return;
   }
@@ -434,7 +432,7 @@ void initialiseMDSM(SURVEY *survey) {
   g.maxshift = greatest;
   // nsamp: as given, else sized from device memory (two chunks in flight:
   // float and storage input rows and outputs per beam), capped at 2^20
-  const double sb = (double)storage_bytes(g.storage);
+  const double sb = (double)mdsm::storage_bytes(g.storage);
   if (survey->nsamp == 0) {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) tot = (size_t)8 << 30;
@@ -485,7 +483,7 @@ void initialiseMDSM(SURVEY *survey) {
     }
   }
   const size_t in_bytes = (size_t)g.nbeams * g.nchans * g.ld * 4;
-  const size_t st_bytes = (size_t)g.nbeams * g.nchans * g.ld * storage_bytes(g.storage);
+  const size_t st_bytes = (size_t)g.nbeams * g.nchans * g.ld * mdsm::storage_bytes(g.storage);
   const size_t out_bytes = (size_t)g.nbeams * g.tdms * g.nsamp * 4;
   for (int p = 0; p < 2; ++p) {
     if (!hip_ok(hipMalloc((void **)&g.d_raw[p], in_bytes), "input buffer") ||

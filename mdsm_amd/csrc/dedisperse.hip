@@ -24,12 +24,18 @@
 //        round-to-nearest-even.  (v_dot2_f32_f16 would add two halves per
 //        instruction but is not correctly rounded: it drops low bits of
 //        the exact sum in cancellation and wide-exponent cases.)
+//   F16_PAIRS  the F16 layout; for each trial the words of channels 2k and
+//        2k+1 are added with one v_pk_add_f16 (two samples, each one IEEE
+//        FP16 rounding) and the pair sum goes into the FP32 accumulators
+//        with v_fma_mix_f32: three VALU instructions per four adds instead
+//        of four, at about 1.7x the rounding error of F16 against FP64.
 //   U8   2 B per add: bytes widened to pairs of u16 in the same two-copy
 //        layout, summed exactly with v_pk_add_u16 (two adds per
 //        instruction); every <= 256 channels their high bytes move into a
 //        second u16 pair accumulator (sum = 256 * high + low, exact).
 // Results are bit-identical to a sequential FP32 sum in channel order (F32,
-// F16 storage) or to the exact integer sum (U8).  MFMA is not used: the
+// F16 storage), to the FP32 sum of FP16 channel-pair sums (F16_PAIRS) or to
+// the exact integer sum (U8).  MFMA is not used: the
 // operation is shift-and-add, not a contraction.
 //
 // Reference counterpart: shared_dedispersion
@@ -146,6 +152,7 @@ __device__ __forceinline__ const T *at_bytes(const T *p, int32_t bytes) {
 // the compiler.
 template <int MW, int K, int CB_, int BURST = 0>
 struct F32Tile {
+  static constexpr bool PAIRS = false;
   static constexpr int TT = 64 * K;
   static constexpr int TD = 4 * MW;
   static constexpr int CB = CB_;
@@ -293,6 +300,9 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
   using G = PackedGeom<MW, K2, CB_>;
   static constexpr int TT = G::TT, TD = G::TD, CB = CB_;
   static constexpr bool IS_U8 = MODE == MDSM_U8;
+  // F16_PAIRS: channel blocks hold an even number of rows (stage_channels)
+  static constexpr bool PAIRS = MODE == MDSM_F16_PAIRS;
+  static_assert(!PAIRS || CB_ % 2 == 0, "F16_PAIRS tiles stage whole channel pairs");
   using T = std::conditional_t<IS_U8, uint8_t, __half>;
   static bool fits(uint32_t) { return true; }
   // raw staging dwords per row (U8): wp/2 + 2 source dwords, padded to 64
@@ -396,6 +406,13 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
                                                            __builtin_amdgcn_alignbit(c, a.w, 16));
           }
         }
+        // F16_PAIRS, odd block (the last of an odd channel count): row nc,
+        // inside the buffer since cbr is even, becomes the zero partner
+        if (PAIRS && (nc & 1u)) {
+          uint32_t *Z = b + nc * rw;
+          for (uint32_t w = 4u * tid; w < rw; w += 1024u)
+            *reinterpret_cast<uint4 *>(Z + w) = make_uint4(0u, 0u, 0u, 0u);
+        }
       }
     };
 
@@ -472,31 +489,77 @@ struct PackedTile : PackedGeom<MW, K2, CB_> {
           facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
         }
       };
+      if constexpr (PAIRS) {
+        // rows r, r+1 of the block are channels 2k, 2k+1 (blocks start at
+        // even channels); the pair sum rounds once to FP16, then to FP32.
+        // An odd last channel pairs with the zero row build() leaves after
+        // it: h + 0 = h exactly, and the accumulators, which start at +0,
+        // are never -0, so adding +0 for -0 changes no result.
+        auto add_pair = [&](int i, int j, uint32_t a, uint32_t b2) {
+          const half2_t s = __builtin_bit_cast(half2_t, a) + __builtin_bit_cast(half2_t, b2);
+          add_word(i, j, __builtin_bit_cast(uint32_t, s));
+        };
 #pragma unroll
-      for (int r = 0; r < CB; ++r) {
-        if ((uint32_t)r < nc) {
-          if constexpr (BURST > 0) {
+        for (int r = 0; r < CB; r += 2) {
+          if ((uint32_t)r < nc) {
+            if constexpr (BURST > 0) {
 #pragma unroll
-            for (int i0 = 0; i0 < MW; i0 += BURST) {
-              uint32_t w[BURST][K2];
+              for (int i0 = 0; i0 < MW; i0 += BURST) {
+                uint32_t w0[BURST][K2], w1[BURST][K2];
 #pragma unroll
-              for (int g = 0; g < BURST; ++g) {
-                const uint32_t *p = at_bytes(cur, rel.get(r, i0 + g));
+                for (int g = 0; g < BURST; ++g) {
+                  const uint32_t *p0 = at_bytes(cur, rel.get(r, i0 + g));
+                  const uint32_t *p1 = at_bytes(cur, rel.get(r + 1, i0 + g));
 #pragma unroll
-                for (int j = 0; j < K2; ++j) w[g][j] = p[64 * j];
+                  for (int j = 0; j < K2; ++j) {
+                    w0[g][j] = p0[64 * j];
+                    w1[g][j] = p1[64 * j];
+                  }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int g = 0; g < BURST; ++g)
+#pragma unroll
+                  for (int j = 0; j < K2; ++j) add_pair(i0 + g, j, w0[g][j], w1[g][j]);
               }
-              __builtin_amdgcn_sched_barrier(0);
+            } else {
 #pragma unroll
-              for (int g = 0; g < BURST; ++g)
+              for (int i = 0; i < MW; ++i) {
+                const uint32_t *p0 = at_bytes(cur, rel.get(r, i));
+                const uint32_t *p1 = at_bytes(cur, rel.get(r + 1, i));
 #pragma unroll
-                for (int j = 0; j < K2; ++j) add_word(i0 + g, j, w[g][j]);
+                for (int j = 0; j < K2; ++j) add_pair(i, j, p0[64 * j], p1[64 * j]);
+              }
             }
-          } else {
+          }
+        }
+      } else {
 #pragma unroll
-            for (int i = 0; i < MW; ++i) {
-              const uint32_t *p = at_bytes(cur, rel.get(r, i));
+        for (int r = 0; r < CB; ++r) {
+          if ((uint32_t)r < nc) {
+            if constexpr (BURST > 0) {
 #pragma unroll
-              for (int j = 0; j < K2; ++j) add_word(i, j, p[64 * j]);
+              for (int i0 = 0; i0 < MW; i0 += BURST) {
+                uint32_t w[BURST][K2];
+#pragma unroll
+                for (int g = 0; g < BURST; ++g) {
+                  const uint32_t *p = at_bytes(cur, rel.get(r, i0 + g));
+#pragma unroll
+                  for (int j = 0; j < K2; ++j) w[g][j] = p[64 * j];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int g = 0; g < BURST; ++g)
+#pragma unroll
+                  for (int j = 0; j < K2; ++j) add_word(i0 + g, j, w[g][j]);
+              }
+            } else {
+#pragma unroll
+              for (int i = 0; i < MW; ++i) {
+                const uint32_t *p = at_bytes(cur, rel.get(r, i));
+#pragma unroll
+                for (int j = 0; j < K2; ++j) add_word(i, j, p[64 * j]);
+              }
             }
           }
         }
@@ -541,6 +604,14 @@ template <int MW, int K2, int CB, int BURST = 0>
 using F16Tile = PackedTile<MDSM_F16, MW, K2, CB, BURST>;
 template <int MW, int K2, int CB, int BURST = 0>
 using U8Tile = PackedTile<MDSM_U8, MW, K2, CB, BURST>;
+// The F16_PAIRS tile of an F16 tile shape; grouped reads take half as many
+// trials, since each reads two channels.
+template <typename F16T>
+struct PairsOf;
+template <int MW, int K2, int CB, int BURST>
+struct PairsOf<PackedTile<MDSM_F16, MW, K2, CB, BURST>> {
+  using type = PackedTile<MDSM_F16_PAIRS, MW, K2, CB, BURST / 2>;
+};
 
 // XCD-aware workgroup order for the single-beam launch: off unless
 // MDSM_XCD_REMAP=1 (measured at 1024 channels x 4096 trials x 2^20 samples it
@@ -730,6 +801,7 @@ int pick_dtype(int dtype, Fn &&fn) {
     case MDSM_F32: return fn((typename V::F32 *)nullptr);
     case MDSM_F16: return fn((typename V::F16 *)nullptr);
     case MDSM_U8: return fn((typename V::U8 *)nullptr);
+    case MDSM_F16_PAIRS: return fn((typename PairsOf<typename V::F16>::type *)nullptr);
   }
   set_error("unknown input dtype %d", dtype);
   return MDSM_ERR_ARG;
@@ -759,12 +831,14 @@ int with_tile(int dtype, int variant, Fn &&fn) {
 }
 
 // Channels staged per block: the tile's CB, reduced while the block's LDS
-// (two buffers, plus raw staging for U8) exceeds 64 KB (160 KB per CU).
+// (two buffers, plus raw staging for U8) exceeds 64 KB (160 KB per CU); even
+// for F16_PAIRS, whose channel pairs must not straddle two blocks.
 template <typename Tile>
 uint32_t stage_channels(uint32_t rw) {
   const uint64_t per = Tile::lds_words_per_row(rw) * sizeof(float);
+  const uint64_t step = Tile::PAIRS ? 2 : 1;
   uint64_t n = (uint64_t)Tile::CB;
-  while (n > 1 && n * per > 64 * 1024) --n;
+  while (n > step && n * per > 64 * 1024) n -= step;
   if (n * per > 160 * 1024) return 0;
   return (uint32_t)n;
 }
@@ -773,7 +847,9 @@ uint32_t stage_channels(uint32_t rw) {
 
 int num_variants(int) { return kNumVariants; }
 
-int preferred_variant(int dtype) { return dtype == MDSM_F32 ? 8 : (dtype == MDSM_F16 ? 15 : 2); }
+int preferred_variant(int dtype) {
+  return dtype == MDSM_F32 ? 8 : (storage_dtype(dtype) == MDSM_F16 ? 15 : 2);
+}
 
 TileShape variant_shape(int dtype, int variant) {
   TileShape sh{0, 0, 0};
@@ -798,7 +874,7 @@ uint32_t lds_row_words(int dtype, int variant, uint32_t spread) {
 }
 
 uint32_t input_alignment(int dtype) {
-  return dtype == MDSM_F32 ? 1u : (dtype == MDSM_F16 ? 2u : 4u);
+  return dtype == MDSM_F32 ? 1u : (storage_dtype(dtype) == MDSM_F16 ? 2u : 4u);
 }
 
 uint32_t stage_channels(int dtype, int variant, uint32_t rw) {

@@ -657,7 +657,7 @@ int launch_normalise(const float *raw, uint64_t ld, uint32_t nchans, uint64_t n,
                      const double *mean, const double *stdv, void *out, int out_dtype,
                      uint64_t ldo, hipStream_t s) {
   const dim3 grid(grid_for(n, NT * 8, 4096), nchans);
-  switch (out_dtype) {
+  switch (storage_dtype(out_dtype)) {
     case MDSM_F32:
       hipLaunchKernelGGL(normalise_kernel<MDSM_F32>, grid, dim3(NT), 0, s, raw, ld, n, mean, stdv, out, ldo);
       break;

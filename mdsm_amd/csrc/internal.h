@@ -32,6 +32,14 @@ void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Input types: storage element of each (F16_PAIRS rows are F16 rows).
+inline bool known_dtype(int dtype) { return dtype >= MDSM_F32 && dtype <= MDSM_F16_PAIRS; }
+inline int storage_dtype(int dtype) { return dtype == MDSM_F16_PAIRS ? MDSM_F16 : dtype; }
+inline size_t storage_bytes(int dtype) {
+  const int st = storage_dtype(dtype);
+  return st == MDSM_F32 ? 4u : (st == MDSM_F16 ? 2u : 1u);
+}
+
 // ---------------------------------------------------------------- tiles
 // A dedispersion tile variant: every wave owns MW consecutive DM trials and
 // K time samples per lane (lane + 64*j), a workgroup of 4 waves owns
@@ -53,7 +61,7 @@ uint32_t tile_spread(const int32_t *table, uint32_t nchans, uint32_t ndms, int t
 // LDS words per staged channel row of a variant for a delay spread; 0 when
 // the variant cannot stage that spread.
 uint32_t lds_row_words(int dtype, int variant, uint32_t spread);
-// Required multiple (in elements) of the input row length: 1 (F32), 2 (F16), 4 (U8).
+// Required multiple (in elements) of the input row length: 1 (F32), 2 (F16, F16_PAIRS), 4 (U8).
 uint32_t input_alignment(int dtype);
 
 // Validates survey parameters and fills the host delay table.

@@ -37,7 +37,6 @@ struct mdsm_pipeline_s {
   hipStream_t graph_stream = nullptr;
 };
 
-static size_t isize(int dt) { return dt == MDSM_F32 ? 4 : (dt == MDSM_F16 ? 2 : 1); }
 
 static void drop_graph(mdsm_pipeline_s *p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -92,14 +91,14 @@ extern "C" int mdsm_pipeline_create(const mdsm_survey_t *beams, uint32_t nbeams,
     mdsm_batch_info(p->batch, b, nullptr, &min_ld);
     p->ld_store[b] = min_ld;
     const size_t nb = (s.nsamp + dp->stats_block - 1) / dp->stats_block;
-    ok = hipMalloc(&p->d_store[b], s.nchans * min_ld * isize(dtype)) == hipSuccess &&
+    ok = hipMalloc(&p->d_store[b], s.nchans * min_ld * storage_bytes(dtype)) == hipSuccess &&
          hipMalloc((void **)&p->d_out[b], (size_t)s.ndms * s.nsamp * 4) == hipSuccess &&
          hipMalloc((void **)&p->d_cmean[b], s.nchans * 8) == hipSuccess &&
          hipMalloc((void **)&p->d_cstd[b], s.nchans * 8) == hipSuccess &&
          hipMalloc((void **)&p->d_bmean[b], s.ndms * nb * 8) == hipSuccess &&
          hipMalloc((void **)&p->d_bstd[b], s.ndms * nb * 8) == hipSuccess &&
          hipMalloc((void **)&p->d_cands[b], (size_t)dp->cand_cap * sizeof(mdsm_cand_t)) == hipSuccess;
-    if (ok) ok = hipMemset(p->d_store[b], 0, s.nchans * min_ld * isize(dtype)) == hipSuccess;
+    if (ok) ok = hipMemset(p->d_store[b], 0, s.nchans * min_ld * storage_bytes(dtype)) == hipSuccess;
   }
   if (!ok) {
     set_error("pipeline allocation failed");

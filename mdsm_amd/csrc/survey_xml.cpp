@@ -296,6 +296,7 @@ SURVEY *processSurveyParameters(const std::string &filepath) {
         if (t == "fp32" || t == "f32" || t == "float32") s->input_storage = MDSM_F32;
         else if (t == "fp16" || t == "f16" || t == "float16" || t == "half") s->input_storage = MDSM_F16;
         else if (t == "u8" || t == "uint8") s->input_storage = MDSM_U8;
+        else if (t == "fp16pairs" || t == "f16pairs" || t == "fp16_pairs") s->input_storage = MDSM_F16_PAIRS;
         else {
           g_xml_err = "unknown storage type: " + *v;
           free(s);

@@ -68,6 +68,7 @@ _sig("oracle_calculate_nsamp_ref", _u32, _u32, _u32, _u32, _u32, _u64)
 _sig("oracle_dedisperse_f64", None, _f32p, _i64, _i, _i, _i, _i32p, _f64p)
 _sig("oracle_dedisperse_f32seq", None, _f32p, _i64, _i, _i, _i, _i32p, _f32p)
 _sig("oracle_dedisperse_f16", None, _u16p, _i64, _i, _i, _i, _i32p, _f32p)
+_sig("oracle_dedisperse_f16_pairs", None, _u16p, _i64, _i, _i, _i, _i32p, _f32p)
 _sig("oracle_dedisperse_u8", None, _u8p, _i64, _i, _i, _i, _i32p, _f32p)
 _sig("oracle_f32_to_f16_array", None, _f32p, _u16p, _i64)
 _sig("oracle_f16_to_f32_array", None, _u16p, _f32p, _i64)
@@ -216,6 +217,15 @@ def dedisperse_f16(h, table, nsamp):
     ndms = table.shape[1]
     out = np.empty((ndms, nsamp), np.float32)
     _lib.oracle_dedisperse_f16(h, ld, nchans, nsamp, ndms, _c(table, np.int32), out)
+    return out
+
+
+def dedisperse_f16_pairs(h, table, nsamp):
+    h = _c(h, np.uint16)
+    nchans, ld = h.shape
+    ndms = table.shape[1]
+    out = np.empty((ndms, nsamp), np.float32)
+    _lib.oracle_dedisperse_f16_pairs(h, ld, nchans, nsamp, ndms, _c(table, np.int32), out)
     return out
 
 

@@ -65,6 +65,11 @@
  *                      spikes and plateaus, DM > 1 rule, hand-derived
  *                      per-chunk statistics cases
  *   f16 conversion     numpy float16 (round to nearest even)
+ *   dedisperse_f16_pairs  a numpy restatement that rounds the exact float64
+ *                      pair sum straight to float16 (no float32 step),
+ *                      ties-to-even known answers (2048 + 1 -> 2048,
+ *                      1 + 2^-11 -> 1), odd channel counts, exact integer
+ *                      sums
  *   bandpass / RFI     exact sums on integer data, float64 means within
  *                      float rounding, hand-built bursts and spikes whose
  *                      flagged blocks and clipped spectra are known; the
@@ -432,6 +437,35 @@ void oracle_dedisperse_f16(const uint16_t *in, int64_t ld, int nchans, int nsamp
     for (int c = 0; c < nchans; ++c) {
       const uint16_t *r = in + (int64_t)c * ld + table[(int64_t)c * ndms + d];
       for (int t = 0; t < nsamp; ++t) o[t] += oracle_f16_to_f32(r[t]);
+    }
+  }
+}
+
+/* FP16-stored input summed in channel pairs (the library's MDSM_F16_PAIRS;
+ * no reference counterpart: a storage-precision extension of the same sum,
+ * multibeam_dedispersion_kernel.cu:44-95).  For k = 0, 1, ...:
+ * s = fl16(h[2k] + h[2k+1]) (no partner past the last channel: s = h[2k]),
+ * out = fl32(out + s).  The float sum of two halves rounds to 24 bits, and
+ * 24 >= 2*11 + 2, so rounding it again to FP16 gives the correctly rounded
+ * FP16 sum of the exact value (double rounding is innocuous at that
+ * precision; sums in the FP16 subnormal range are exact in both). */
+void oracle_dedisperse_f16_pairs(const uint16_t *in, int64_t ld, int nchans, int nsamp, int ndms,
+                                 const int32_t *table, float *out) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int d = 0; d < ndms; ++d) {
+    float *o = out + (int64_t)d * nsamp;
+    for (int t = 0; t < nsamp; ++t) o[t] = 0.0f;
+    for (int c = 0; c < nchans; c += 2) {
+      const uint16_t *r0 = in + (int64_t)c * ld + table[(int64_t)c * ndms + d];
+      if (c + 1 < nchans) {
+        const uint16_t *r1 = in + (int64_t)(c + 1) * ld + table[(int64_t)(c + 1) * ndms + d];
+        for (int t = 0; t < nsamp; ++t) {
+          const float f = oracle_f16_to_f32(r0[t]) + oracle_f16_to_f32(r1[t]);
+          o[t] += oracle_f16_to_f32(oracle_f32_to_f16(f));
+        }
+      } else {
+        for (int t = 0; t < nsamp; ++t) o[t] += oracle_f16_to_f32(r0[t]);
+      }
     }
   }
 }

@@ -1,7 +1,7 @@
 """GPU parity of the multi-beam dedisperser (persistent queue and flat
 XCD-ordered launch): every beam of an uneven batch (different nchans,
-nsamp, ndms, dmstep, band) must equal the oracle bit for bit, for F32, F16
-and U8 storage."""
+nsamp, ndms, dmstep, band) must equal the oracle bit for bit, for F32, F16,
+F16 channel-pair sums and U8 storage."""
 import sys
 
 import numpy as np
@@ -13,11 +13,11 @@ from mdsm_amd import abi
 pytestmark = pytest.mark.gpu
 
 
-def _beams(seed, n):
+def _beams(seed, n, odd=False):
     rng = np.random.default_rng(seed)
     out = []
-    for _ in range(n):
-        C = int(rng.choice([32, 64, 96, 128]))
+    for k in range(n):
+        C = int(rng.choice([32, 64, 96, 128])) - (1 if odd and k % 2 else 0)
         N = int(rng.integers(1000, 9000))
         M = int(rng.integers(10, 140))
         step = float(rng.uniform(0.1, 0.5))
@@ -27,9 +27,10 @@ def _beams(seed, n):
     return out
 
 
-@pytest.mark.parametrize("dtype", [abi.F32, abi.F16, abi.U8])
+@pytest.mark.parametrize("dtype", [abi.F32, abi.F16, abi.U8, abi.F16_PAIRS])
 def test_batch_uneven_beams_bitexact(dtype):
-    beams = _beams(100 + dtype, 9)
+    # the pair sums also get odd channel counts (a zero partner row)
+    beams = _beams(100 + dtype, 9, odd=dtype == abi.F16_PAIRS)
     batch = abi.Batch(beams, dtype)
     print(f"[batch dtype {dtype}] {len(beams)} beams", flush=True, file=sys.stderr)
     rng = np.random.default_rng(dtype)
@@ -44,6 +45,9 @@ def test_batch_uneven_beams_bitexact(dtype):
         elif dtype == abi.F16:
             h = O.f32_to_f16(x)
             refs.append(O.dedisperse_f16(h, t, s.nsamp))
+        elif dtype == abi.F16_PAIRS:
+            h = O.f32_to_f16(x)
+            refs.append(O.dedisperse_f16_pairs(h, t, s.nsamp))
         else:
             h = rng.integers(0, 256, x.shape).astype(np.uint8)
             refs.append(O.dedisperse_u8(h, t, s.nsamp))

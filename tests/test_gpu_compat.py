@@ -410,14 +410,15 @@ def test_streaming_manager_file_rotation(tmp_path):
         assert len(lines) == len(cands[i][0]) > 0
 
 
-@pytest.mark.parametrize("storage", ["fp16", "u8"])
+@pytest.mark.parametrize("storage", ["fp16", "u8", "fp16pairs"])
 def test_streaming_manager_reduced_storage(tmp_path, storage):
-    """<storage type="fp16"/> or "u8": each chunk's rows (overlap tail and
+    """<storage type="fp16"/>, "u8" or "fp16pairs" (FP16 rows, channel pairs
+    added in FP16 before the FP32 sum): each chunk's rows (overlap tail and
     new samples) are normalised per channel over their valid samples, then
     quantised; the returned series equal the oracle's dedispersion of the
     same quantised rows bit for bit, and the candidates its threshold."""
     C_, N, M, S = _open(tmp_path, _plain(storage=f'<storage type="{storage}" />'))
-    dt = {"fp16": abi.F16, "u8": abi.U8}[storage]
+    dt = {"fp16": abi.F16, "u8": abi.U8, "fp16pairs": abi.F16_PAIRS}[storage]
     assert lib.mdsm_compat_storage() == dt
     table = O.delay_table_ref(1500.0, -1.0, C_, 64e-6, 0.0, 1.0, M)
     gain, offset = O.channel_bandpass(36, C_)
@@ -435,7 +436,8 @@ def test_streaming_manager_reduced_storage(tmp_path, storage):
         row[:, S:S + n] = stream[:, col:col + n]
         col += n
         m, sd = O.channel_stats(row, S + n)
-        q = O.normalise(row, m, sd, {"fp16": "f16", "u8": "u8"}[storage])
-        want = O.dedisperse_f16(q, table, N) if storage == "fp16" else O.dedisperse_u8(q, table, N)
+        q = O.normalise(row, m, sd, {"fp16": "f16", "u8": "u8", "fp16pairs": "f16"}[storage])
+        want = {"fp16": O.dedisperse_f16, "u8": O.dedisperse_u8,
+                "fp16pairs": O.dedisperse_f16_pairs}[storage](q, table, N)
         assert np.array_equal(got[k][:, :n], want[:, :n]), k
         _check_cands(cands[k], np.ascontiguousarray(want[:, :n]), 0.0, 64e-6)

@@ -3,15 +3,16 @@
 * The 32-beam uneven batch (nchans 512/1024, nsamp 2^17..2^20, ndms
   1024..4096, dmstep 0.1..0.5, 1500 MHz top over 300 MHz, 64 us; seeded
   noise, bandpass and 20 pulses per beam, generated and bandpass-normalised
-  on the device) dedispersed by one batched launch with FP32, FP16 and U8
-  storage, in both launch shapes (persistent queues and the flat grid):
+  on the device) dedispersed by one batched launch with FP32, FP16, FP16
+  with channel-pair sums and U8 storage, in both launch shapes (persistent
+  queues and the flat grid):
   - sampled full-length DM rows of four beams equal the oracle of the same
     storage bit for bit, and the candidate lists of those rows (device
     detector) equal the oracle's;
   - FP16 and U8 rows against the FP64 sum of the unquantised normalised
-    input: the error is printed for every beam checked, and FP16 stays
-    within 1e-3 of the series maximum; FP32 and FP16 candidate lists equal
-    those of the FP64 rows;
+    input: the error is printed for every beam checked, and FP16 (also
+    with pair sums) stays within 1e-3 of the series maximum; FP32 and FP16
+    candidate lists equal those of the FP64 rows;
   - sampled rows of every beam agree between the two launch shapes;
   - with a channel-constant input (x[c][t] = c mod 7) every sampled row of
     every beam is the exact channel sum, whatever the delays.
@@ -34,8 +35,9 @@ from mdsm_amd import abi
 
 pytestmark = pytest.mark.gpu
 
-ISZ = {abi.F32: 4, abi.F16: 2, abi.U8: 1}
-NAME = {abi.F32: "f32", abi.F16: "f16", abi.U8: "u8"}
+ISZ = {abi.F32: 4, abi.F16: 2, abi.U8: 1, abi.F16_PAIRS: 2}
+NAME = {abi.F32: "f32", abi.F16: "f16", abi.U8: "u8", abi.F16_PAIRS: "f16_pairs"}
+NPDT = {abi.F32: np.float32, abi.F16: np.uint16, abi.U8: np.uint8, abi.F16_PAIRS: np.uint16}
 
 
 def log(msg):
@@ -68,6 +70,8 @@ def _oracle_rows(dtype, x, table, nsamp):
         return O.dedisperse_f32seq(x, t, nsamp)
     if dtype == abi.F16:
         return O.dedisperse_f16(x, t, nsamp)
+    if dtype == abi.F16_PAIRS:
+        return O.dedisperse_f16_pairs(x, t, nsamp)
     return O.dedisperse_u8(x, t, nsamp)
 
 
@@ -92,7 +96,7 @@ def _fill_channel_constant(ins, surveys, dtype):
     """x[c][t] = c mod 7 in every beam's input rows (device-to-device copies
     from seven pattern rows)."""
     ldmax = max(ld for _, ld in ins)
-    pat_np = {abi.F32: np.float32, abi.F16: np.float16, abi.U8: np.uint8}[dtype]
+    pat_np = {abi.F32: np.float32, abi.F16: np.float16, abi.U8: np.uint8, abi.F16_PAIRS: np.float16}[dtype]
     pat = np.repeat(np.arange(7, dtype=pat_np)[:, None], ldmax, axis=1)
     dpat = abi.DeviceBuffer.from_numpy(pat)
     isz = ISZ[dtype]
@@ -111,7 +115,8 @@ def _channel_constant_expect(dtype, C):
     return np.float32(v.sum())
 
 
-@pytest.mark.parametrize("dtype", [abi.F32, abi.F16, abi.U8], ids=["f32", "f16", "u8"])
+@pytest.mark.parametrize("dtype", [abi.F32, abi.F16, abi.U8, abi.F16_PAIRS],
+                         ids=["f32", "f16", "u8", "f16_pairs"])
 def test_batch_32_beams_full_size(batch32, dtype):
     beams, surveys, outs, checked = batch32
     batch = abi.Batch(surveys, dtype)
@@ -145,8 +150,7 @@ def test_batch_32_beams_full_size(batch32, dtype):
         ld = ins[b][1]
         rows = bench.precision_rows(beams[b])
         table, _ = abi.delay_table(s)
-        x = ins[b][0].download({abi.F32: np.float32, abi.F16: np.uint16, abi.U8: np.uint8}[dtype],
-                               (s.nchans, ld))
+        x = ins[b][0].download(NPDT[dtype], (s.nchans, ld))
         got = _rows(outs[b], s, rows)
         ref = _oracle_rows(dtype, x, table[:, rows], s.nsamp)
         assert np.array_equal(got, ref), (NAME[dtype], b, rows)
@@ -175,11 +179,11 @@ def test_batch_32_beams_full_size(batch32, dtype):
         log(f"[cfg2/3 {NAME[dtype]}] beam {b} (C={s.nchans} N={s.nsamp} M={s.ndms} step={s.dmstep:.3f}) "
             f"rows {rows}: bit-exact vs oracle, {on} candidates; vs FP64: max err {e_max:.2e} of max, "
             f"{e_rms:.2e} of rms; candidates differing from the FP64 rows' {ndiff}")
-        if dtype == abi.F16:
+        if dtype in (abi.F16, abi.F16_PAIRS):
             assert e_max <= 1e-3, e_max
         if dtype in (abi.F32, abi.F16):
             assert ndiff == 0
-        else:
+        if dtype == abi.U8:
             assert e_max <= 3e-2, e_max
 
     # channel-constant known answer on every beam (flat launch)

@@ -92,11 +92,16 @@ def test_f16_u8_bitexact_and_error():
     u = O.normalise(x, m, sd, "u8")
     y8 = abi.dedisperse_host(s, u, abi.U8)
     assert np.array_equal(y8, O.dedisperse_u8(u, t, 16384))
+    y16p = abi.dedisperse_host(s, h, abi.F16_PAIRS)
+    assert np.array_equal(y16p, O.dedisperse_f16_pairs(h, t, 16384))
     # errors against FP64 of the unquantised input, reported
     y64 = O.dedisperse_f64(x, t, 16384)
     mx, rel = _err_stats(y16, y64)
     _say(f"    f16 storage: max|y-y64|/rms = {rel:.3e}")
     assert rel < 2e-3
+    mxp, relp = _err_stats(y16p, y64)
+    _say(f"    f16 pair sums: max|y-y64|/rms = {relp:.3e}")
+    assert relp < 4e-3
 
 
 @pytest.mark.parametrize("shape", [(37, 45, 1000, 2.5, 1.7), (130, 129, 777, 0.0, 9.0), (600, 70, 3001, 0.0, 0.3)])
@@ -113,9 +118,20 @@ def test_packed_ragged_shapes_all_variants(shape):
     h = O.f32_to_f16(rng.standard_normal((C, N + S + 5)).astype(np.float32))
     u = rng.integers(0, 256, (C, N + S + 5)).astype(np.uint8)
     ref16 = O.dedisperse_f16(h, t, N)
+    ref16p = O.dedisperse_f16_pairs(h, t, N)
     ref8 = O.dedisperse_u8(u, t, N)
+    p16, p16p = abi.Plan(s, abi.F16), abi.Plan(s, abi.F16_PAIRS)
     for v in range(1, abi.tile_variants() + 1):
         assert np.array_equal(abi.dedisperse_host(s, h, abi.F16, v), ref16), ("f16", v)
+        if p16p.variant_channels(v) == 0:
+            # a channel pair per LDS block does not fit this delay spread,
+            # where F16 stages one row at a time: a clean error
+            assert p16.variant_channels(v) <= 1, v
+            with pytest.raises(abi.MdsmError):
+                abi.dedisperse_host(s, h, abi.F16_PAIRS, v)
+        else:
+            assert p16p.variant_channels(v) % 2 == 0, v
+            assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref16p), ("f16 pairs", v)
         assert np.array_equal(abi.dedisperse_host(s, u, abi.U8, v), ref8), ("u8", v)
 
 

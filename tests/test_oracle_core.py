@@ -234,6 +234,50 @@ def test_dedisperse_f16_and_u8():
     assert np.array_equal(O.dedisperse_u8(u, table, N), expect)
 
 
+def _pairs_numpy(h, table, N):
+    """Channel-pair sums restated with numpy: the exact pair sum in float64
+    (two halves never need more than 41 significant bits), rounded straight
+    to float16, then float32 adds in pair order."""
+    C, M = table.shape
+    idx = np.arange(N)[None, None, :] + table[:, :, None]
+    g = h.view(np.float16)[np.arange(C)[:, None, None], idx].astype(np.float64)  # [C][M][N]
+    acc = np.zeros((M, N), np.float32)
+    for c in range(0, C, 2):
+        s = g[c] + g[c + 1] if c + 1 < C else g[c]
+        acc += s.astype(np.float16).astype(np.float32)
+    return acc
+
+
+@pytest.mark.parametrize("C", [32, 31, 1, 2])
+def test_dedisperse_f16_pairs_vs_numpy(C):
+    x, table, N = _small_case(C=max(C, 2), seed=11)
+    x, table = x[:C], table[:C]
+    h = O.f32_to_f16(x * 3)
+    assert np.array_equal(O.dedisperse_f16_pairs(h, table, N), _pairs_numpy(h, table, N))
+
+
+def test_dedisperse_f16_pairs_known_answers():
+    # two channels, DM 0 only: ties of the FP16 pair sum go to even, where
+    # the FP32 channel-order sum of the same halves keeps the low bit
+    table = np.zeros((2, 1), np.int32)
+    cases = [(2048.0, 1.0, 2048.0, 2049.0),               # ulp 2 at 2048: tie -> 2048
+             (1.0, 2.0 ** -11, 1.0, 1.0 + 2.0 ** -11),    # half an ulp of 1: tie -> 1
+             (1.0, 3 * 2.0 ** -11, 1.0 + 2.0 ** -9, 1.0 + 3 * 2.0 ** -11),  # 1.5 ulp -> 2 ulp
+             (2.0 ** -24, 2.0 ** -24, 2.0 ** -23, 2.0 ** -23),  # subnormals: exact
+             (65504.0, 16.0, np.inf, 65520.0)]            # tie at the overflow edge -> inf
+    for a, b, pair, seq in cases:
+        h = O.f32_to_f16(np.array([[a], [b]], np.float32))
+        assert O.dedisperse_f16_pairs(h, table, 1)[0, 0] == np.float32(pair), (a, b)
+        assert O.dedisperse_f16(h, table, 1)[0, 0] == np.float32(seq), (a, b)
+    # integer-valued halves with exact pair sums: equal to the exact sum
+    x, table, N = _small_case(C=33, seed=4)
+    xi = np.round(x * 8).astype(np.float32)
+    h = O.f32_to_f16(xi)
+    idx = np.arange(N)[None, None, :] + table[:, :, None]
+    exact = xi[np.arange(33)[:, None, None], idx].astype(np.int64).sum(0)
+    assert np.array_equal(O.dedisperse_f16_pairs(h, table, N), exact.astype(np.float32))
+
+
 # ---------------------------------------------------------------- statistics
 def test_channel_stats_and_normalise():
     rng = np.random.default_rng(1)
