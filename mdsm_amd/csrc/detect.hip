@@ -250,7 +250,9 @@ __global__ __launch_bounds__(NT) void detect_kernel(const float *__restrict__ x,
 // lane's own run come from the next lane (h <= 16) or the lane after it
 // (h = 32) by wave shuffles, so a level costs no LDS round trip and no
 // barrier.  A wave covers 64 runs; after seven levels the runs of lanes
-// 0..56 are complete, so a wave owns DW_SEG = 57 * 16 samples and reads the
+// 0..56 are complete (a level-k value of lane i depends on lanes up to
+// i + 1, 2, 3, 4, 5, 7 for k = 1..6, and lane 63's missing neighbour only
+// reaches lanes 57..63), so a wave owns DW_SEG = 57 * 16 samples and reads the
 // next 112 again (12 % extra reads, mostly L2 hits).  Each run lies inside
 // one statistics block (L % 16 == 0), so a lane forms its block's w * mean
 // and threshold bounds once per level instead of once per sample.  Same
@@ -259,6 +261,15 @@ __global__ __launch_bounds__(NT) void detect_kernel(const float *__restrict__ x,
 constexpr int DW_R = 16;
 constexpr int DW_LANES = 57;
 constexpr int DW_SEG = DW_LANES * DW_R;
+
+// The value of lane + 1 (0 in lane 63): DPP wave_shl:1 on both halves, a
+// VALU move instead of a shuffle through the LDS crossbar.
+__device__ __forceinline__ double from_next_lane(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x130, 0xF, 0xF, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x130, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 
 __global__ __launch_bounds__(NT) void detect_wave_kernel(
     const float *__restrict__ x, uint32_t nsamp, uint32_t L, uint32_t nb,
@@ -305,9 +316,15 @@ __global__ __launch_bounds__(NT) void detect_wave_kernel(
     const double den = sd * kSqrtW[k];
     const bool in_range = den > 0x1p-960 && den < 0x1p960;
     const double bound = in_range ? lim * den : -INFINITY;
+    // One compare per sample: fl(v - wm) >= bound needs v >= wm + bound -
+    // 2^-52 |bound| (the subtraction's rounding), and vt lies below that by a
+    // margin of 2^-48 (|wm| + |bound|), which covers the roundings of vt's
+    // own two operations; so no sample that passes the exact test below is
+    // filtered out here.
+    const double vt = in_range ? (wm + bound) - (fabs(wm) + fabs(bound)) * 0x1p-48 : -INFINITY;
     bool any = false;
 #pragma unroll
-    for (int i = 0; i < DW_R; ++i) any |= v[i] - wm >= bound;
+    for (int i = 0; i < DW_R; ++i) any |= v[i] >= vt;
     if (live && any) {
       // (a is formed again here, so the common path holds no array of it)
 #pragma unroll
@@ -334,14 +351,17 @@ __global__ __launch_bounds__(NT) void detect_wave_kernel(
       double nx[DW_R / 2];
 #pragma unroll
       for (int j = 0; j < DW_R / 2; ++j)
-        if (j < h) nx[j] = __shfl_down(v[j], 1);
+        if (j < h) nx[j] = from_next_lane(v[j]);
 #pragma unroll
       for (int i = 0; i < DW_R; ++i) v[i] = v[i] + (i + h < DW_R ? v[i + h] : nx[i + h - DW_R]);
     } else {
       // h = 16: the same element of the next lane; h = 32: of the lane after
       // it (every lane reads the old value before adding)
 #pragma unroll
-      for (int i = 0; i < DW_R; ++i) v[i] = v[i] + __shfl_down(v[i], h / DW_R);
+      for (int i = 0; i < DW_R; ++i) {
+        const double n1 = from_next_lane(v[i]);
+        v[i] = v[i] + (h == DW_R ? n1 : from_next_lane(n1));
+      }
     }
     test_level(k);
   }
