@@ -334,21 +334,24 @@ COMPAT_XML = """<observation>
   <samples nsamp="{nsamp}" nbits="32" />
   <detection detectionThreshold="6" />
   <storage type="{storage}" />
+  <series download="{download}" />
   <writer filePrefix="compat" baseDirectory="{dir}" singleFileMode="1" />
   <beams><beam beamId="0" topFrequency="1500" frequencyOffset="-0.29296875" /></beams>
 </observation>"""
 
 
-def compat_throughput(nsamp=1 << 18, ndms=4096, nchunks=10, storage="fp32"):
+def compat_throughput(nsamp=1 << 18, ndms=4096, nchunks=10, storage="fp32", download=True):
     """The drop-in manager API (get_buffer_pointer / next_chunk_multibeam /
     start_processing) driven as src/MDSM.cpp drives it, host buffers in and
     out: per chunk the input upload (PCIe), the overlap-save copy,
-    dedispersion, the global-statistics threshold detection and the
-    download of the dedispersed series into the caller's output buffers
-    (PCIe).  The manager pipelines chunks across three streams, so the
-    rate is taken over the whole stream: from the call of data chunk 3
-    (every staging buffer filled once by then) to the return of the final,
-    empty start_processing call that drains the pipeline."""
+    dedispersion, the global-statistics threshold detection and, with
+    download, the download of the dedispersed series into the caller's
+    output buffers (PCIe; without it the series stay on the device, which
+    src/MDSM.cpp, ignoring next_chunk_multibeam's return value, allows).
+    The manager pipelines chunks across three streams, so the rate is taken
+    over the whole stream: from the call of data chunk 3 (every staging
+    buffer filled once by then) to the return of the final, empty
+    start_processing call that drains the pipeline."""
     import ctypes as C
     import tempfile
     lib = abi.lib()
@@ -362,7 +365,8 @@ def compat_throughput(nsamp=1 << 18, ndms=4096, nchunks=10, storage="fp32"):
     with tempfile.TemporaryDirectory() as d:
         xml = os.path.join(d, "obs.xml")
         with open(xml, "w") as f:
-            f.write(COMPAT_XML.format(ndms=ndms, nsamp=nsamp, dir=d, storage=storage))
+            f.write(COMPAT_XML.format(ndms=ndms, nsamp=nsamp, dir=d, storage=storage,
+                                      download=1 if download else 0))
         if lib.mdsm_compat_open(xml.encode(), 0) != 0:
             raise RuntimeError(lib.mdsm_compat_last_error().decode())
         nb, nch, ns, nd, ms = (C.c_uint() for _ in range(5))
@@ -392,11 +396,11 @@ def compat_throughput(nsamp=1 << 18, ndms=4096, nchunks=10, storage="fp32"):
         t_end = time.perf_counter()
         lib.mdsm_compat_close()
     per = (t_end - t_first) / (nchunks - 2)  # chunks 3..nchunks and the drain
+    pcie = Cn * N * 4 + (M * N * 4 if download else 0)
     return {"workload": f"1 beam, 1024 ch x {M} DMs (step 0.2) x {N} samples per chunk, host buffers, "
-                        f"{storage} storage",
+                        f"{storage} storage, series {'downloaded' if download else 'kept on the device'}",
             "s_per_chunk": per, "dm_samples_per_s": M * N / per,
-            "pcie_bytes_per_chunk": Cn * N * 4 + M * N * 4,
-            "pcie_gbs": (Cn * N * 4 + M * N * 4) / per / 1e9, "chunks_timed": nchunks - 2}
+            "pcie_bytes_per_chunk": pcie, "pcie_gbs": pcie / per / 1e9, "chunks_timed": nchunks - 2}
 
 
 def detect_run(steps, rows=2048, nsamp=1 << 20):
@@ -497,6 +501,8 @@ def main():
         return
     if args.compat:
         res = {st: compat_throughput(storage=st) for st in ("fp32", "fp16")}
+        for st in ("fp32", "fp16", "fp16pairs"):
+            res[st + "_series_on_device"] = compat_throughput(storage=st, download=False)
         log(f"[compat] {res}")
         print(json.dumps({"compat_host_buffers": res}))
         return

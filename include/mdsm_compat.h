@@ -99,6 +99,12 @@ typedef struct {
    * reference's), MDSM_F16 (1), MDSM_U8 (2) or MDSM_F16_PAIRS (3), from
    * <storage type="fp32|fp16|u8|fp16pairs"/>. */
   unsigned input_storage;
+  /* Extension: 1 keeps the dedispersed series on the device, so that
+   * next_chunk_multibeam returns NULL and the chunk's series cross no bus
+   * (callers such as src/MDSM.cpp ignore them; detection and the candidate
+   * files are unchanged); 0, the reference's behaviour, downloads them into
+   * the returned buffers.  From <series download="0|1"/>. */
+  unsigned series_on_device;
 } SURVEY;
 
 /* XML dialect recorded by processSurveyParameters. */

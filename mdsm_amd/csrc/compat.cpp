@@ -17,7 +17,8 @@
 //          median filter / detrending;
 //   s_out  chunk k: the reference detector (global FP64 mean/rms and
 //          threshold) on the device, then the download of the dedispersed
-//          series into the pinned output ring.
+//          series into the pinned output ring (skipped, with no ring, when
+//          SURVEY::series_on_device is set).
 // Device inputs and outputs are double-buffered by chunk parity, host
 // outputs form a ring of three (a returned buffer stays valid until the
 // next next_chunk_multibeam call).  start_processing(k) enqueues chunk k and
@@ -511,7 +512,7 @@ void initialiseMDSM(SURVEY *survey) {
                               (size_t)g.nbeams * g.nsamp * g.nchans * sizeof(float)),
                 "pinned input"))
       return;
-  for (int k = 0; k < NRING; ++k) {
+  for (int k = 0; k < NRING && !survey->series_on_device; ++k) {
     g.out_ring[k] = (float **)calloc(g.nbeams, sizeof(float *));
     for (unsigned i = 0; i < g.nbeams; ++i) {
       if (!hip_ok(hipHostMalloc((void **)&g.out_ring[k][i], (size_t)g.tdms * g.nsamp * sizeof(float)),
@@ -571,7 +572,7 @@ float **next_chunk_multibeam(unsigned int data_read, unsigned &samples, double t
     return nullptr;
   }
   samples = g.ppnsamp;
-  return g.out_ring[slot];
+  return g.out_ring[slot];  // NULL when the series stay on the device
 }
 
 int start_processing(unsigned int data_read) {
@@ -655,7 +656,7 @@ int start_processing(unsigned int data_read) {
     if (!enqueue_detect(p, b, data_read, data_read, g.s_out)) return 0;
   if (!hip_ok(hipEventRecord(g.ev_det[p], g.s_out), "event")) return 0;
   const int slot = k % NRING;
-  for (unsigned b = 0; b < g.nbeams; ++b)
+  for (unsigned b = 0; b < g.nbeams && g.out_ring[slot]; ++b)
     if (!hip_ok(hipMemcpyAsync(g.out_ring[slot][b], out_ptr(p, b), (size_t)g.tdms * N * 4,
                                hipMemcpyDeviceToHost, g.s_out), "output download"))
       return 0;

@@ -303,6 +303,20 @@ SURVEY *processSurveyParameters(const std::string &filepath) {
           return nullptr;
         }
       }
+    } else if (n == "series") {
+      // extension: whether next_chunk_multibeam returns the dedispersed
+      // series (no reference counterpart, so it marks neither dialect)
+      auto it = e->attrs.find("download");
+      if (it != e->attrs.end()) {
+        const std::string &v = it->second;
+        if (v == "0" || v == "false") s->series_on_device = 1;
+        else if (v == "1" || v == "true") s->series_on_device = 0;
+        else {
+          g_xml_err = "series download must be 0 or 1: " + v;
+          free(s);
+          return nullptr;
+        }
+      }
     } else if (n == "gpus") {
       if (auto v = a.get("gpuIDs", "ids")) {
         auto ids = split_uints(*v);

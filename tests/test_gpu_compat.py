@@ -276,7 +276,8 @@ def _drive(stream, C_, S, N, M, reads, ts=lambda i: 0.0, block=64e-6, before_sto
         samples = C.c_uint()
         outs = lib.mdsm_compat_next_chunk(n, C.byref(samples), ts(i), block)
         if i >= 3 and i - 2 <= len(reads):
-            got[i - 2] = np.ctypeslib.as_array(outs[0], shape=(M, N)).copy()
+            # NULL when the series stay on the device
+            got[i - 2] = np.ctypeslib.as_array(outs[0], shape=(M, N)).copy() if outs else None
         r = lib.mdsm_compat_start(n)
         if i >= 2:
             cands[i - 1] = _candidates()
@@ -441,3 +442,31 @@ def test_streaming_manager_reduced_storage(tmp_path, storage):
                 "fp16pairs": O.dedisperse_f16_pairs}[storage](q, table, N)
         assert np.array_equal(got[k][:, :n], want[:, :n]), k
         _check_cands(cands[k], np.ascontiguousarray(want[:, :n]), 0.0, 64e-6)
+
+
+def test_streaming_manager_series_on_device(tmp_path):
+    """<series download="0"/>: next_chunk_multibeam returns NULL once it
+    would return series, no series cross the bus, and every chunk's
+    candidates equal those of the same stream with the series downloaded
+    (and the oracle's threshold on the oracle's series)."""
+    reads = [4096, 4096, 4096]
+    runs = {}
+    for dl in (1, 0):
+        d = tmp_path / f"dl{dl}"
+        d.mkdir()
+        C_, N, M, S = _open(d, _plain().replace("</observation>", f'  <series download="{dl}" />\n</observation>'))
+        table = O.delay_table_ref(1500.0, -1.0, C_, 64e-6, 0.0, 1.0, M)
+        stream = O.generate(44, C_, 1500.0, -1.0, 64e-6, 0, S + sum(reads),
+                            pulses=[(20.0, 1500, 8, 1.5), (12.0, N + 2500, 8, 1.5), (30.0, 2 * N + 900, 8, 1.5)])
+        runs[dl] = _drive(stream, C_, S, N, M, reads, ts=lambda i: 5.0 * i)
+        lib.mdsm_compat_close()
+    got1, c1 = runs[1]
+    got0, c0 = runs[0]
+    assert all(g is not None for g in got1.values()) and all(g is None for g in got0.values())
+    assert sorted(c1) == sorted(c0)
+    for k in sorted(c1):
+        for a, b in zip(c1[k], c0[k]):
+            assert np.array_equal(a, b), k
+    full = O.dedisperse_f32seq(stream, table, 3 * N)
+    for k in (1, 2, 3):
+        _check_cands(c0[k], np.ascontiguousarray(full[:, (k - 1) * N:k * N]), 5.0 * k, 64e-6)
