@@ -20,7 +20,8 @@ def _beams():
     return out
 
 
-@pytest.mark.parametrize("dtype,name", [(abi.F16, "f16"), (abi.U8, "u8"), (abi.F32, "f32")])
+@pytest.mark.parametrize("dtype,name", [(abi.F16, "f16"), (abi.U8, "u8"), (abi.F32, "f32"),
+                                        (abi.F16_PAIRS, "f16_pairs")])
 def test_pipeline_eager_and_graph_match_oracle(dtype, name):
     beams = _beams()
     pipe = abi.Pipeline(beams, dtype, stats_block=4096, nwidths=7, thresh=5.0, cand_cap=1 << 18)
@@ -39,8 +40,9 @@ def test_pipeline_eager_and_graph_match_oracle(dtype, name):
         # up to the storage alignment, within the raw row), normalise, dedisperse, detect
         n = min(ld, -(-ld // abi.ALIGN[dtype]) * abi.ALIGN[dtype])
         m, sd = O.channel_stats(x, n)
-        xs = O.normalise(x, m, sd, name, n)
-        y = {"f32": O.dedisperse_f32seq, "f16": O.dedisperse_f16, "u8": O.dedisperse_u8}[name](xs, t, s.nsamp)
+        xs = O.normalise(x, m, sd, "f16" if name == "f16_pairs" else name, n)
+        y = {"f32": O.dedisperse_f32seq, "f16": O.dedisperse_f16, "u8": O.dedisperse_u8,
+             "f16_pairs": O.dedisperse_f16_pairs}[name](xs, t, s.nsamp)
         refs.append((y, O.detect(y.astype(np.float64), 4096, 7, 5.0)))
     pipe.bind([p for p, _ in raws], [l for _, l in raws])
 
