@@ -12,6 +12,7 @@
 // only a counter-clear kernel and the persistent kernel, so it can be
 // captured in a hipGraph and replayed per chunk.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -40,6 +41,17 @@ struct mdsm_batch_s {
   uint32_t max_row_words = 0;
   std::vector<BeamDesc> h_desc;
 };
+
+// Adjacent time tiles per XCD group (the item order is described at
+// decode_item in dedisperse.hip).  Measured on the 32-beam batch: 4 cuts the
+// input fetched from beyond L2 from 35.7x to 19.9x the input bytes (2:
+// 23.3x; 8 overflows L2: 34.6x) and runs 0-2 % faster than 1; MDSM_TGROUP
+// (1..16) overrides it.
+static uint32_t batch_tgroup() {
+  const char *e = getenv("MDSM_TGROUP");
+  const int a = e ? atoi(e) : 4;
+  return a >= 1 && a <= 16 ? (uint32_t)a : 4u;
+}
 
 // A tile shape can run the batch when every beam's delay spread has an LDS
 // row (row_words > 0) and the LDS holds at least one staged channel of the
@@ -299,9 +311,10 @@ extern "C" int mdsm_batch_bind(mdsm_batch batch, const void *const *d_in, const 
     d.n_ttiles = (s.nsamp + sh.tt() - 1) / sh.tt();
     d.n_dtiles = (s.ndms + sh.td() - 1) / sh.td();
     d.item_begin = (uint32_t)items;
-    d.pad = 0;
-    // items are numbered XCD-major: 8 slots per group of 8 time tiles
-    items += (uint64_t)(d.n_ttiles + 7u) / 8u * 8u * d.n_dtiles;
+    d.tgroup = batch_tgroup();
+    // items are numbered XCD-major: 8 slots per group of 8 * tgroup time tiles
+    const uint32_t span = 8u * d.tgroup;
+    items += (uint64_t)(d.n_ttiles + span - 1u) / span * span * d.n_dtiles;
   }
   MDSM_REQUIRE(items < (1ull << 31), MDSM_ERR_ARG, "too many work items");
   MDSM_HIP(hipMemcpy(batch->d_desc, batch->h_desc.data(), sizeof(BeamDesc) * nb,

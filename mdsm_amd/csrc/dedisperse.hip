@@ -677,10 +677,13 @@ __global__ __launch_bounds__(256) void trial_offset_kernel(const int32_t *__rest
 }
 
 // Multi-beam item numbering (both launch shapes): a beam's items are
-// XCD-major, 8 * ceil(n_ttiles / 8) * n_dtiles of them starting at a multiple
-// of 8; item g belongs to XCD slot g % 8 and decodes to time tile
-// 8 * (k / n_dtiles) + g % 8, DM tile k % n_dtiles, k = (g - begin) / 8
-// (padding items with a time tile past the beam's end do nothing).
+// XCD-major, 8 A * ceil(n_ttiles / (8 A)) * n_dtiles of them starting at a
+// multiple of 8, A = tgroup; item g belongs to XCD slot s = g % 8, and with
+// k = (g - begin) / 8, j = k / (A n_dtiles), r = k % (A n_dtiles) it is DM
+// tile r / A of time tile 8 A j + A s + r % A: each XCD takes groups of A
+// adjacent time tiles and runs a DM tile for all A of them back to back, so
+// their shifted windows overlap in its L2 (A = 1: time tile 8 j + s, DM
+// tile r).  Padding items with a time tile past the beam's end do nothing.
 struct BeamItem {
   const BeamDesc *bd;
   uint32_t tt, dt;
@@ -697,11 +700,13 @@ __device__ __forceinline__ BeamItem decode_item(const BeamDesc *__restrict__ bea
   }
   const BeamDesc *bd = beams + lo;
   const uint32_t local = g - bd->item_begin;
-  const uint32_t k = local >> 3;
+  const uint32_t k = local >> 3, A = bd->tgroup;
+  const uint32_t per = A * bd->n_dtiles;
+  const uint32_t j = k / per, r = k - j * per;
   BeamItem it;
   it.bd = bd;
-  it.tt = ((k / bd->n_dtiles) << 3) + (local & 7u);
-  it.dt = k % bd->n_dtiles;
+  it.dt = r / A;
+  it.tt = 8u * A * j + A * (local & 7u) + (r - it.dt * A);
   it.valid = it.tt < bd->n_ttiles;
   return it;
 }

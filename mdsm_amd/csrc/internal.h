@@ -96,7 +96,7 @@ struct BeamDesc {
   uint32_t nchans, nsamp, ndms, row_words;
   uint32_t n_ttiles, n_dtiles;
   uint32_t item_begin;  // first work item of this beam in the queue order
-  uint32_t pad;
+  uint32_t tgroup;      // adjacent time tiles per XCD group (1: every 8th time tile)
 };
 
 int launch_batch_flat(int dtype, int variant, const BeamDesc *d_beams, uint32_t nbeams,
