@@ -468,6 +468,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-beam FP32 pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the batched FP16 launches")
+    ap.add_argument("--profile-pairs", action="store_true",
+                    help="with --profile: the batched launches with FP16 channel-pair sums")
     ap.add_argument("--variant", type=int, default=None,
                     help="tile variant of the batched FP16 launch instead of the autotuner's "
                          "(for profiling one tile shape)")
@@ -605,14 +607,17 @@ def main():
             capture_rows(dtype, ins)
         return batch, ins, ms, B, v
 
-    batch16, ins16, ms16, B16, v16 = run_batch(abi.F16, "fp16", v=args.variant)
+    # the headline storage (in --profile runs optionally the channel-pair sums)
+    hdt = abi.F16_PAIRS if (args.profile and args.profile_pairs) else abi.F16
+    batch16, ins16, ms16, B16, v16 = run_batch(hdt, "fp16" if hdt == abi.F16 else "fp16 pairs",
+                                               v=args.variant)
     modes16 = batch16.mode_ms
     checks = {}
     t0tab, _ = abi.delay_table(surveys[0])
-    checks["beam0"] = spot_check(beams[0], t0tab, ins16[0][0], ins16[0][1], abi.F16, outs[0],
+    checks["beam0"] = spot_check(beams[0], t0tab, ins16[0][0], ins16[0][1], hdt, outs[0],
                                  [0, surveys[0].ndms // 2, surveys[0].ndms - 1])
     tl, _ = abi.delay_table(surveys[-1])
-    checks["beam_last"] = spot_check(beams[-1], tl, ins16[-1][0], ins16[-1][1], abi.F16, outs[-1],
+    checks["beam_last"] = spot_check(beams[-1], tl, ins16[-1][0], ins16[-1][1], hdt, outs[-1],
                                      [1, surveys[-1].ndms - 2])
     log(f"[batch fp16] spot checks {checks}")
     det = {"candidates": 0, "clusters": 0, "pulses_found": 0, "pulses": 0}
@@ -777,7 +782,7 @@ def main():
     achieved = B16 / (ms16 / 1e3)
     traffic = None
     traffic_source = None
-    if os.path.exists(TRAFFIC_FILE):  # measured HBM bytes per launch (rocprofv3 --pmc)
+    if hdt == abi.F16 and os.path.exists(TRAFFIC_FILE):  # measured HBM bytes per launch (rocprofv3 --pmc)
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
         # counters of another tile shape do not describe this launch
@@ -791,7 +796,8 @@ def main():
             log(f"no traffic profile of tile variant {v16} (profiled: {sorted(tr.get('by_variant', {}))}): "
                 "roofline.traffic left null")
     line = {
-        "metric": "dedispersed DM x samples per second (batched, FP16 storage)",
+        "metric": "dedispersed DM x samples per second (batched, FP16 storage)" if hdt == abi.F16 else
+                  "dedispersed DM x samples per second (batched, FP16 storage, channel-pair sums)",
         "value": value,
         "unit": "DM*samples/s",
         "n_gpus": 1,
@@ -802,7 +808,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "storage": "f16",
+        "storage": "f16" if hdt == abi.F16 else "f16_pairs",
         "data": "synthetic (seeded Philox noise, bandpass, 20 injected pulses per beam), generated on the device",
         "config": {"workload": f"{len(beams)} uneven beams, nchans 512/1024, nsamp 2^17..2^20, "
                                "ndms 1024..4096, dmstep 0.1..0.5, one launch",
