@@ -99,6 +99,33 @@ def hybrid_rule(prob, events, stats, t, fp_fraction=0.5):
     return min(ok) if ok else SNR_KEEP[0]
 
 
+def recall_fp_wide(prob, events, stats, t, wide_snr, wide_k=6):
+    """As recall_fp, for the rule that keeps an event when the network's
+    probability exceeds t or the event is found at the widest boxcar
+    (width_log2 >= wide_k) with S/N >= wide_snr."""
+    keep = (prob > t) | ((events["width_log2"] >= wide_k) & (events["snr"] >= wide_snr))
+    hits = keep & (events["label"] == 1)
+    found = len(set(zip(events["chunk"][hits].tolist(), events["pulse"][hits].tolist())))
+    return found / max(stats["injected"], 1), int(((events["label"] == 0) & keep).sum())
+
+
+WIDE_SNR_KEEP = (1e9, 12.0, 10.0, 9.0, 8.0, 7.5, 7.0, 6.5, 6.0)
+
+
+def wide_rule(prob, events, stats, fp_fraction=0.5):
+    """(threshold, widest-boxcar S/N rescue level) of the highest recall on a
+    validation stream whose false positives stay under fp_fraction of the
+    plain path's (ties: fewer false positives)."""
+    budget = fp_fraction * stats["false_clusters"]
+    best = None
+    for t in THRESHOLDS:
+        for w in WIDE_SNR_KEEP:
+            r, f = recall_fp_wide(prob, events, stats, t, w)
+            if f <= budget and (best is None or (r, -f) > (best[2], -best[3])):
+                best = (t, w, r, f)
+    return (best[0], best[1]) if best else (THRESHOLDS[0], WIDE_SNR_KEEP[0])
+
+
 def missed_pulses(prob, events, t):
     """Features of the best event (highest S/N) of every injected pulse that
     the plain path found but no event of which the network keeps at t."""
@@ -270,6 +297,11 @@ def main(argv=None):
     snr_keep = hybrid_rule(pv, ev_va, st_va, thr_half)
     res["hybrid_snr_keep"] = snr_keep
     log(f"hybrid rule: prob > {thr_half} or S/N >= {snr_keep}")
+    # the widest-boxcar rescue at the half budget: threshold and rescue level
+    # chosen together on the validation stream
+    thr_wide, wide_snr = wide_rule(pv, ev_va, st_va)
+    res["wide_rule"] = {"threshold": thr_wide, "wide_snr_keep": wide_snr}
+    log(f"wide rule: prob > {thr_wide} or (widest boxcar and S/N >= {wide_snr})")
     log(f"operating thresholds {thr} (quarter of the plain false positives), {thr_half} (half)")
     del d_va, d_prob
     thresholds = THRESHOLDS
@@ -296,16 +328,18 @@ def main(argv=None):
         r["operating_point_half"] = r[f"cnn@{thr_half}"]
         hr, hf = recall_fp_rule(prob, ev_te, st_te, thr_half, snr_keep)
         r["hybrid_half"] = {"recall": hr, "false_positives": hf, "snr_keep": snr_keep}
+        wr, wf = recall_fp_wide(prob, ev_te, st_te, thr_wide, wide_snr)
+        r["wide_half"] = {"recall": wr, "false_positives": wf, "threshold": thr_wide, "wide_snr_keep": wide_snr}
         r["missed_at_half"] = missed_pulses(prob, ev_te, thr_half)
         r["inference_events_per_s"] = n / max(t_inf, 1e-9)
         res["eval"].append(r)
         log(json.dumps({k: r[k] for k in ("plain", "operating_point", "operating_point_half", "hybrid_half",
-                                          "missed_at_half")}))
+                                          "wide_half", "missed_at_half")}))
     if args.cfg2_chunks > 0:
         # held-out streams over the 32 beam shapes of the batch, aggregated
         # (configs: inference on the batch's streams, the plain path judging)
         agg = {"plain_found": 0, "injected": 0, "plain_fp": 0, "clusters": 0, "events": 0,
-               "cnn_found": {}, "cnn_fp": {}, "hyb_found": 0, "hyb_fp": 0}
+               "cnn_found": {}, "cnn_fp": {}, "hyb_found": 0, "hyb_fp": 0, "wide_found": 0, "wide_fp": 0}
         t_all = time.time()
         for b, sv in enumerate(cfg2_surveys()):
             cap = 2000 * args.cfg2_chunks
@@ -326,6 +360,9 @@ def main(argv=None):
             hr, hf = recall_fp_rule(prob, ev_te, st_te, thr_half, snr_keep)
             agg["hyb_found"] += round(hr * st_te["injected"])
             agg["hyb_fp"] += hf
+            wr, wf = recall_fp_wide(prob, ev_te, st_te, thr_wide, wide_snr)
+            agg["wide_found"] += round(wr * st_te["injected"])
+            agg["wide_fp"] += wf
             for t in sorted(set((thr, thr_half) + THRESHOLDS)):
                 keep = prob > t
                 hits = keep & (ev_te["label"] == 1)
@@ -344,10 +381,12 @@ def main(argv=None):
         c2["operating_point_half"] = c2[f"cnn@{thr_half}"]
         c2["hybrid_half"] = {"recall": agg["hyb_found"] / max(agg["injected"], 1),
                              "false_positives": agg["hyb_fp"], "snr_keep": snr_keep}
+        c2["wide_half"] = {"recall": agg["wide_found"] / max(agg["injected"], 1),
+                           "false_positives": agg["wide_fp"], "threshold": thr_wide, "wide_snr_keep": wide_snr}
         res["eval_batch_shapes"] = c2
         log(json.dumps({"batch_shapes": {k: c2[k] for k in ("plain", "operating_point",
                                                            "operating_point_half", "hybrid_half",
-                                                           "injected")}}))
+                                                           "wide_half", "injected")}}))
     if args.save:
         np.save(args.save, net.params())
     if args.json:
