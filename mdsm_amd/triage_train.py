@@ -191,7 +191,7 @@ def oversample_positives(d_snip, events, k, k_wide=1, log=print):
     return d, labels, n
 
 
-def cfg2_surveys(seed=2026):
+def batch_beam_surveys(seed=2026):
     """The beam shapes of the 32-beam uneven batch (nchans 512/1024, ndms
     1024-4096, dmstep 0.1-0.5, 1500 MHz top over 300 MHz, 64 us), drawn as
     the benchmark draws them, each searched in 2^17-sample chunks."""
@@ -235,7 +235,7 @@ def main(argv=None):
                     help="pulse events repeated this many times in the training set (class weight)")
     ap.add_argument("--wide-weight", type=int, default=1,
                     help="extra repetition of pulse events found at the two widest boxcars")
-    ap.add_argument("--cfg2-chunks", type=int, default=4,
+    ap.add_argument("--batch-beam-chunks", type=int, default=4,
                     help="chunks per beam of the held-out stream over the 32 batch beam shapes (0: skip)")
     ap.add_argument("--save", default=None, help="write trained parameters (.npy, canonical layout)")
     ap.add_argument("--json", default=None, help="write the results as JSON")
@@ -335,16 +335,16 @@ def main(argv=None):
         res["eval"].append(r)
         log(json.dumps({k: r[k] for k in ("plain", "operating_point", "operating_point_half", "hybrid_half",
                                           "wide_half", "missed_at_half")}))
-    if args.cfg2_chunks > 0:
+    if args.batch_beam_chunks > 0:
         # held-out streams over the 32 beam shapes of the batch, aggregated
         # (configs: inference on the batch's streams, the plain path judging)
         agg = {"plain_found": 0, "injected": 0, "plain_fp": 0, "clusters": 0, "events": 0,
                "cnn_found": {}, "cnn_fp": {}, "hyb_found": 0, "hyb_fp": 0, "wide_found": 0, "wide_fp": 0}
         t_all = time.time()
-        for b, sv in enumerate(cfg2_surveys()):
-            cap = 2000 * args.cfg2_chunks
+        for b, sv in enumerate(batch_beam_surveys()):
+            cap = 2000 * args.batch_beam_chunks
             d_te, ev_te, st_te, _ = build_stream(make_config(sv), args.seed + 1000 + b, cap,
-                                                 max_chunks=args.cfg2_chunks, log=lambda m: None)
+                                                 max_chunks=args.batch_beam_chunks, log=lambda m: None)
             if len(ev_te) >= cap:
                 raise RuntimeError("batch-shape stream filled its event buffer; raise the capacity")
             n = len(ev_te)
@@ -370,7 +370,7 @@ def main(argv=None):
                 agg["cnn_found"][t] = agg["cnn_found"].get(t, 0) + found
                 agg["cnn_fp"][t] = agg["cnn_fp"].get(t, 0) + int(((ev_te["label"] == 0) & keep).sum())
             del d_te, d_prob
-        c2 = {"beams": 32, "chunks_per_beam": args.cfg2_chunks, "seconds": time.time() - t_all,
+        c2 = {"beams": 32, "chunks_per_beam": args.batch_beam_chunks, "seconds": time.time() - t_all,
               "plain": {"recall": agg["plain_found"] / max(agg["injected"], 1),
                         "false_positives": agg["plain_fp"], "clusters": agg["clusters"]},
               "injected": agg["injected"]}

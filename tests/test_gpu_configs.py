@@ -130,14 +130,14 @@ def test_batch_32_beams_full_size(batch32, dtype):
         ins.append((d, ld))
     abi.synchronize()
     args = ([d.ptr for d, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
-    log(f"[cfg2/3 {NAME[dtype]}] {len(beams)} beams generated and normalised on the device")
+    log(f"[32-beam batch {NAME[dtype]}] {len(beams)} beams generated and normalised on the device")
 
     sampled = {}
     for mode in (abi.BATCH_PERSISTENT, abi.BATCH_FLAT):
         for o in outs:
             o.zero()
         batch.set_mode(mode)
-        log(f"[cfg2/3 {NAME[dtype]}] launch mode {mode}")
+        log(f"[32-beam batch {NAME[dtype]}] launch mode {mode}")
         batch.dedisperse(*args)
         abi.synchronize()
         sampled[mode] = [_rows(outs[b], s, _sample_rows(s)) for b, s in enumerate(surveys)]
@@ -176,7 +176,7 @@ def test_batch_32_beams_full_size(batch32, dtype):
         e_rms = float(err.max() / np.sqrt((y64 * y64).mean()))
         fd, ft, fk, _, _ = O.detect(y64, 16384, 7, 6.0)
         ndiff = len(_cand_set(od, ot, ok) ^ _cand_set(fd, ft, fk))
-        log(f"[cfg2/3 {NAME[dtype]}] beam {b} (C={s.nchans} N={s.nsamp} M={s.ndms} step={s.dmstep:.3f}) "
+        log(f"[32-beam batch {NAME[dtype]}] beam {b} (C={s.nchans} N={s.nsamp} M={s.ndms} step={s.dmstep:.3f}) "
             f"rows {rows}: bit-exact vs oracle, {on} candidates; vs FP64: max err {e_max:.2e} of max, "
             f"{e_rms:.2e} of rms; candidates differing from the FP64 rows' {ndiff}")
         if dtype in (abi.F16, abi.F16_PAIRS):
@@ -195,7 +195,7 @@ def test_batch_32_beams_full_size(batch32, dtype):
     for b, s in enumerate(surveys):
         got = _rows(outs[b], s, _sample_rows(s))
         assert (got == _channel_constant_expect(dtype, s.nchans)).all(), (NAME[dtype], b)
-    log(f"[cfg2/3 {NAME[dtype]}] channel-constant known answer on {len(surveys)} beams")
+    log(f"[32-beam batch {NAME[dtype]}] channel-constant known answer on {len(surveys)} beams")
     st.free()
     for d, _ in ins:
         d.free()
@@ -208,7 +208,7 @@ def test_single_beam_full_search():
     M, N = s.ndms, s.nsamp
     plan = abi.Plan(s, abi.F32)
     ld = plan.min_ld
-    log(f"[cfg1] C=1024 N=2^20 M=4096 maxshift={plan.maxshift}")
+    log(f"[single beam] C=1024 N=2^20 M=4096 maxshift={plan.maxshift}")
     d_in = abi.DeviceBuffer(s.nchans * ld * 4)
     d_out = abi.DeviceBuffer(M * N * 4)
     st = bench.Stage(None)
@@ -227,7 +227,7 @@ def test_single_beam_full_search():
     assert len(c) < cap
     lab, nc = abi.dbscan2d(c, 3, 4)
     peaks, _ = abi.cluster_peaks(c, lab, nc)
-    log(f"[cfg1] {len(c)} candidates, {nc} clusters")
+    log(f"[single beam] {len(c)} candidates, {nc} clusters")
 
     # sampled rows (first, last, the rows of every fifth pulse) vs the oracle
     table, _ = abi.delay_table(s)
@@ -256,8 +256,8 @@ def test_single_beam_full_search():
             missed.append((p.dm, p.start, p.width, p.amp * np.sqrt(1024 * p.width),
                            [(int(q["dm_idx"]), int(q["sample"]), float(q["snr"])) for q in near]))
     for m in missed:
-        log(f"[cfg1] missed pulse DM {m[0]:.2f} start {m[1]} width {m[2]} injected S/N {m[3]:.1f}; "
+        log(f"[single beam] missed pulse DM {m[0]:.2f} start {m[1]} width {m[2]} injected S/N {m[3]:.1f}; "
             f"nearest peak (dm_idx, sample, snr) {m[4]}")
-    log(f"[cfg1] recovered {20 - len(missed)}/20 pulses")
+    log(f"[single beam] recovered {20 - len(missed)}/20 pulses")
     assert len(missed) <= 1
     plan.close()

@@ -40,11 +40,11 @@ def _err_stats(y, y64):
 
 
 @pytest.mark.parametrize("variant", list(range(1, abi.tile_variants() + 1)))
-def test_f32_cfg0_bitexact(variant):
+def test_f32_256ch_bitexact(variant):
     s = _survey(65536)
     t = _table(s)
     S = int(t.max())
-    _say(f"[f32 cfg0 variant {variant}] C=256 N=65536 M=256 S={S}")
+    _say(f"[f32 256-ch variant {variant}] C=256 N=65536 M=256 S={S}")
     x = O.generate(1, 256, s.fch1, s.foff, s.tsamp, 0, 65536 + S)
     y = abi.dedisperse_host(s, x, abi.F32, variant)
     ref = O.dedisperse_f32seq(x, t, 65536)

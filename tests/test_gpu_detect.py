@@ -130,16 +130,16 @@ def _gpu_search(s, x, L=16384, thr=6.0):
     return y, c, peaks
 
 
-def test_cfg0_pipeline_matches_fp64_oracle():
+def test_256ch_pipeline_matches_fp64_oracle():
     """Single beam, C=256, N=65536, M=256 (DM 0..25.5 step 0.1), N(0,1) noise
     and three width-8 pulses of +0.5 per channel at DM 5.0, 12.3, 20.7."""
     s = abi.Survey.make(418.0, -0.078125, 256, 51.2e-6, 0.0, 0.1, 256, 65536)
     t, S = abi.delay_table(s)
     pulses = [(5.0, 9000, 8, 0.5), (12.3, 30000, 8, 0.5), (20.7, 52000, 8, 0.5)]
     x = O.generate(2024, 256, s.fch1, s.foff, s.tsamp, 0, 65536 + S, pulses=pulses)
-    _say("[cfg0 pipeline] GPU search")
+    _say("[256-ch pipeline] GPU search")
     y, c, peaks = _gpu_search(s, x)
-    _say("[cfg0 pipeline] FP64 oracle search")
+    _say("[256-ch pipeline] FP64 oracle search")
     y64 = O.dedisperse_f64(x, t, 65536)
     od, ot, ok, osnr, on = O.detect(y64, 16384, 7, 6.0)
     assert _cands_tuple(c) == list(zip(od.tolist(), ot.tolist(), ok.tolist()))
