@@ -103,6 +103,42 @@ def test_detect_row_tails_and_widths(N, L, nw):
         assert np.array_equal(c["snr"], osnr)
 
 
+@pytest.mark.parametrize("L", [16, 15])
+def test_detect_threshold_boundary(L):
+    """S/N at the threshold to within rounding: one planted sample x per
+    block, every other sample 0, and caller block statistics with the block
+    mean m = x - thr * sd moved by -2..+2 ulps, so the exact test
+    (x - m) / sd >= thr falls on both sides.  Candidates and S/N equal the
+    oracle's on both detector forms (L = 16: register-resident; 15: LDS)."""
+    rng = np.random.default_rng(L)
+    M, nblocks, thr = 5, 96, 4.0
+    N = L * nblocks
+    y = np.zeros((M, N), np.float32)
+    m = np.zeros((M, nblocks))
+    sd = np.zeros((M, nblocks))
+    for r in range(M):
+        for b in range(nblocks):
+            x = np.float32(rng.uniform(1.0, 3.0))
+            s = float(rng.uniform(0.05, 0.2))
+            y[r, b * L + int(rng.integers(0, L))] = x
+            mb = float(x) - thr * s
+            for _ in range(abs((r + b) % 5 - 2)):
+                mb = np.nextafter(mb, np.inf if (r + b) % 5 > 2 else -np.inf)
+            m[r, b], sd[r, b] = mb, s
+    d_y = abi.DeviceBuffer.from_numpy(y)
+    d_m, d_s = abi.DeviceBuffer.from_numpy(m), abi.DeviceBuffer.from_numpy(sd)
+    cap = 1 << 14
+    d_c, d_n = abi.DeviceBuffer(cap * 16), abi.DeviceBuffer(4)
+    abi.detect(d_y.ptr, M, N, L, d_m.ptr, d_s.ptr, 7, thr, d_c.ptr, cap, d_n.ptr)
+    c = abi.read_candidates(d_c, d_n, cap)
+    od, ot, ok, osnr, on = O.detect(y.astype(np.float64), L, 7, thr, stats=(m, sd))
+    _say(f"[detect boundary L={L}] {on} of {M * nblocks} planted samples at or above {thr}")
+    assert 0 < on < M * nblocks
+    assert len(c) == on
+    assert _cands_tuple(c) == list(zip(od.tolist(), ot.tolist(), ok.tolist()))
+    assert np.array_equal(c["snr"], osnr)
+
+
 def test_detect_overflow_is_reported():
     s, t, x, y = _dedispersed(12, 64, 32, 8192)
     d_y = abi.DeviceBuffer.from_numpy(y)
