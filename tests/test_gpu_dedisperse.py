@@ -135,6 +135,35 @@ def test_packed_ragged_shapes_all_variants(shape):
         assert np.array_equal(abi.dedisperse_host(s, u, abi.U8, v), ref8), ("u8", v)
 
 
+@pytest.mark.parametrize("C", [2, 3, 10])
+def test_f16_pairs_rounding_cases_all_variants(C):
+    """The FP16 pair sums at their rounding edges: ties to even (2048 + 1,
+    1 + 2^-11, 1 + 3*2^-11), the overflow tie (65504 + 16 -> inf, and its
+    negative), subnormal sums, exact cancellation, next to random halves;
+    odd channel counts pair the last channel with zero.  Every tile shape
+    equals the oracle bit for bit."""
+    N, M = 1024, 3
+    s = _survey(N, nchans=C, ndms=M, lowdm=0.0, dmstep=0.5, foff=-0.078125 * 256 / C / 4)
+    t = _table(s)
+    S = int(t.max())
+    rng = np.random.default_rng(C)
+    x = (rng.standard_normal((C, N + S + 4)) * 3).astype(np.float32)
+    edge = [(2048.0, 1.0), (1.0, 2.0 ** -11), (1.0, 3 * 2.0 ** -11), (65504.0, 16.0), (-65504.0, -16.0),
+            (2.0 ** -24, 2.0 ** -24), (-2.0 ** -14, 2.0 ** -24), (5.5, -5.5), (1000.0, 0.25)]
+    for e, (a, b) in enumerate(edge):  # planted along DM 0 (rows 0 and 1 at the same column)
+        col = 7 + 97 * e
+        x[0, col + t[0, 0]] = a
+        x[1, col + t[1, 0]] = b
+    h = O.f32_to_f16(x)
+    ref = O.dedisperse_f16_pairs(h, t, N)
+    _say(f"[f16 pairs edges] C={C} M={M} N={N} S={S}")
+    p = abi.Plan(s, abi.F16_PAIRS)
+    for v in range(1, abi.tile_variants() + 1):
+        if p.variant_channels(v) == 0:
+            continue
+        assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref), v
+
+
 def test_u8_near_saturated_sums_all_variants():
     """Bytes of 250..255 over 1500 channels: sums up to 382500 need the
     high-part accumulator after every 256-channel window, in every variant."""
