@@ -49,10 +49,12 @@
  *                      integer data, injected pulse recovered at its DM/width
  *   dbscan2d           scikit-learn DBSCAN(metric="chebyshev")
  *   dbscan_ref         hand-derived cases from the reference algorithm's
- *                      text, and scikit-learn DBSCAN where the neighbour
- *                      graph is a union of cliques (the reference needs Qt
- *                      headers: unbuildable here; chained neighbourhoods are
- *                      pinned only by the hand cases)
+ *                      text, scikit-learn DBSCAN where the neighbour graph
+ *                      is a union of cliques, and on chained random-walk
+ *                      neighbourhoods a closed form: each seed with enough
+ *                      unvisited neighbours takes its connected component
+ *                      among the unvisited points (scipy graph components)
+ *                      (the reference needs Qt headers: unbuildable here)
  *   classify_ref       hand-derived cases (RFI and width rules, the
  *                      histogram smoothing and median), and the whole score
  *                      on clusters whose smoothed DM-S/N curve is the

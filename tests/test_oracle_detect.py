@@ -154,6 +154,66 @@ def test_dbscan_ref_isolated_boxes_match_sklearn(min_points):
     assert np.array_equal(lab, np.where(sk < 0, -1, sk + 1))
 
 
+def _components_rule(tm, dm, snr, ranges, min_points):
+    """The reference's DBSCAN (dbscan.cpp:194-276, neighbours :390-415) in
+    closed form: points are taken as seeds in index order, skipping visited
+    ones; a seed with fewer than min_points unvisited neighbours is noise
+    (visited, and never a neighbour again); otherwise its cluster is the
+    whole connected component that contains it in the neighbour graph
+    restricted to the points not yet visited (every listed point expands,
+    core or not), and all of them become visited.  Neighbours: the float
+    distances |a - b| strictly below the float ranges in all three
+    dimensions.  Components come from scipy's graph labelling, not from a
+    restatement of the reference's marking loop."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    n = len(tm)
+    adj = np.ones((n, n), bool)
+    for v, r in zip((tm, dm, snr), ranges):
+        adj &= np.abs(v[:, None] - v[None, :]).astype(np.float32) < np.float32(r)
+    np.fill_diagonal(adj, False)
+    lab = np.zeros(n, np.int32)
+    visited = np.zeros(n, bool)
+    nclusters = 0
+    for i in range(n):
+        if visited[i]:
+            continue
+        open_ = ~visited
+        if (adj[i] & open_).sum() < min_points:
+            visited[i] = True
+            lab[i] = -1
+            continue
+        idx = np.flatnonzero(open_)
+        _, comp = connected_components(csr_matrix(adj[np.ix_(idx, idx)]), directed=False)
+        members = idx[comp == comp[np.searchsorted(idx, i)]]
+        nclusters += 1
+        lab[members] = nclusters
+        visited[members] = True
+    return lab, nclusters
+
+
+@pytest.mark.parametrize("min_points", [1, 2, 3, 5])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_dbscan_ref_chained_neighbourhoods(min_points, seed):
+    """Random walks in (time, DM, S/N) make chained, branching neighbourhoods
+    in which border points, noise taken early and seeds that fail their
+    count all occur; the oracle equals the component rule on every one."""
+    rng = np.random.default_rng(100 * seed + min_points)
+    ranges = (0.5, 2.0, 1.5)
+    walks = []
+    for w in range(12):
+        steps = rng.uniform(-1.0, 1.0, (int(rng.integers(1, 25)), 3)) * np.array(ranges) * 0.8
+        walks.append(np.array([w * 9.0, (w % 4) * 30.0, 5.0]) + np.cumsum(steps, axis=0))
+    isolated = rng.uniform(0, 1, (15, 3)) * np.array([110.0, 120.0, 20.0])
+    pts = np.concatenate(walks + [isolated])[rng.permutation(sum(map(len, walks)) + 15)]
+    lab, n = O.dbscan_ref(pts[:, 0], pts[:, 1], pts[:, 2], *ranges, min_points)
+    want, nw = _components_rule(pts[:, 0], pts[:, 1], pts[:, 2], ranges, min_points)
+    assert n == nw and np.array_equal(lab, want)
+    # the cases are not trivial: clusters, noise, and clusters of more than
+    # one walk step occur
+    assert n > 2 and (lab == -1).any() and np.bincount(lab[lab > 0]).max() > 3
+
+
 def test_median5_network_is_a_median():
     for perm in itertools.permutations([1.0, 2.0, 3.0, 4.0, 5.0]):
         assert O.median5(*perm) == 3.0
