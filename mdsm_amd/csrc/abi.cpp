@@ -319,7 +319,8 @@ extern "C" int mdsm_plan_create(const mdsm_survey_t *s, int in_dtype, mdsm_plan 
   for (int v = 1; v <= nv; ++v) {
     const TileShape sh = variant_shape(in_dtype, v);
     p->row_words[v] =
-        lds_row_words(in_dtype, v, tile_spread(p->h_table.data(), s->nchans, s->ndms, sh.td()));
+        lds_row_words(in_dtype, v, tile_spread(p->h_table.data(), s->nchans, s->ndms, sh.td()),
+                      p->h_table.data(), s->nchans, s->ndms);
   }
   const size_t bytes = p->h_table.size() * sizeof(int32_t);
   hipError_t e = hipMalloc(&p->d_table, bytes);

@@ -115,7 +115,8 @@ extern "C" int mdsm_batch_create(const mdsm_survey_t *beams, uint32_t nbeams, in
     for (int v = 1; v <= nv; ++v) {
       const TileShape sh = variant_shape(in_dtype, v);
       b->row_words[i][v] = lds_row_words(
-          in_dtype, v, tile_spread(b->h_tables[i].data(), beams[i].nchans, beams[i].ndms, sh.td()));
+          in_dtype, v, tile_spread(b->h_tables[i].data(), beams[i].nchans, beams[i].ndms, sh.td()),
+          b->h_tables[i].data(), beams[i].nchans, beams[i].ndms);
     }
     const size_t bytes = b->h_tables[i].size() * sizeof(int32_t);
     if (hipMalloc(&b->d_tables[i], bytes) != hipSuccess ||

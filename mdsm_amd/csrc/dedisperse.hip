@@ -613,6 +613,318 @@ struct PairsOf<PackedTile<MDSM_F16, MW, K2, CB, BURST>> {
   using type = PackedTile<MDSM_F16_PAIRS, MW, K2, CB, BURST / 2>;
 };
 
+// ====================================================== F16 pair-sum rows
+// F16_PAIRS tiles that add each channel pair once per distinct delay
+// difference instead of once per trial.  For channels c0 = 2k, c1 = 2k+1
+// and trial d, the pair sum fl16(x0[t + D0(d)] + x1[t + D1(d)]) depends on
+// d only through D0(d) and delta(d) = D1(d) - D0(d), and inside one DM tile
+// delta takes a few consecutive values dmin .. dmin+n-1.  After a channel
+// block is staged (one raw window per channel), the build pass adds the
+// pair's windows once per value into sum rows S_j[s] = fl16(x0[s] +
+// x1[s + dmin + j]) (a P and a Q copy, as in the F16 layout), and each trial
+// reads one word of S_{delta(d)-dmin} at D0(d): one LDS word and two
+// v_fma_mix_f32 per four channel adds, where the F16_PAIRS layout above reads
+// two words and adds them with a v_pk_add_f16 for every trial.  The sums are
+// the same IEEE operations on the same operands, so the results are
+// identical to that layout's.  A shape takes a survey when n <= NS and
+// delta >= 0 for every DM tile and pair (pairsum_accepts); the F16_PAIRS
+// forms of the F16 shapes take every survey.
+//
+// LDS: the raw windows of the block's cbr channels (wp words each, one
+// buffer: the build has consumed them before the next block is staged),
+// then NS sum rows of 2 wp words (P, Q) per pair.  The trial loop reads only
+// the sum rows, so the next block's windows can arrive while it runs.
+// Table: the trial offsets (one per pair and trial, relative to the sum
+// rows), then per (DM tile, block, pair) the words {dmin, n}.
+template <int MW, int K2, int CBP, int NS>
+struct PairSumTile {
+  static constexpr int TT = 128 * K2;
+  static constexpr int TD = 4 * MW;
+  static constexpr int CB = 2 * CBP;
+  static constexpr bool PAIRS = true;
+  static constexpr int NSUM = NS;
+  static constexpr int PACK = 1;
+  using T = __half;
+  using Rel = RelOffsets<MW, CBP, PACK>;
+  static_assert(NS >= 1 && NS <= 4, "sum rows per pair (shifts up to 4 samples)");
+  static_assert(CBP == 1 || CBP == 2 || CBP == 4, "waves per pair in the build: 4 / CBP");
+  static bool fits(uint32_t) { return true; }
+  // rw: words of one copy (window of TT + spread samples from an even origin)
+  static uint32_t row_words(uint32_t spread) { return PackedGeom<MW, K2, CB>::copy_words(spread); }
+  // per staged channel: its raw window and half of its pair's sum rows
+  static uint32_t lds_words_per_row(uint32_t rw) { return (1u + (uint32_t)NS) * rw; }
+
+  // The n sum rows of one pair over part of the window: row j is
+  // P_j[k] = fl16(R0[k] + R1 shifted by SH = j + F samples) (two halves per
+  // word, one v_pk_add_f16) and its Q copy, F = dmin & 1.  Lane l builds
+  // words k0 + 4 l .. + 3 of every `step`; R0 and R1 are read once for all
+  // rows.  R0 reads run one word past the row and R1 reads up to four (into
+  // the next staged row or the sum rows, inside the allocation); those words
+  // only reach P and Q words that no trial reads.
+  template <int F>
+  __device__ __forceinline__ static void build_pair(const uint32_t *R0, const uint32_t *R1, uint32_t *S,
+                                                    uint32_t wp, uint32_t n, bool solo, uint32_t k0,
+                                                    uint32_t step) {
+    for (uint32_t k = k0; k < wp; k += step) {
+      const uint4 a = *reinterpret_cast<const uint4 *>(R0 + k);
+      const uint32_t av[5] = {a.x, a.y, a.z, a.w, R0[k + 4]};
+      uint32_t z[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+      if (!solo) {
+        const uint4 z0 = *reinterpret_cast<const uint4 *>(R1 + k);
+        const uint4 z1 = *reinterpret_cast<const uint4 *>(R1 + k + 4);
+        z[0] = z0.x, z[1] = z0.y, z[2] = z0.z, z[3] = z0.w;
+        z[4] = z1.x, z[5] = z1.y, z[6] = z1.z, z[7] = z1.w;
+      }
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        if ((uint32_t)j >= n) break;
+        const int sh = j + F, m = sh >> 1;
+        const bool e = (sh & 1) != 0;
+        uint32_t sv[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+          const uint32_t b = e ? __builtin_amdgcn_alignbit(z[m + i + 1], z[m + i], 16) : z[m + i];
+          const half2_t h = __builtin_bit_cast(half2_t, av[i]) + __builtin_bit_cast(half2_t, b);
+          sv[i] = __builtin_bit_cast(uint32_t, h);
+        }
+        uint32_t *P = S + (uint32_t)j * 2u * wp;
+        *reinterpret_cast<uint4 *>(P + k) = make_uint4(sv[0], sv[1], sv[2], sv[3]);
+        *reinterpret_cast<uint4 *>(P + wp + k) = make_uint4(
+            __builtin_amdgcn_alignbit(sv[1], sv[0], 16), __builtin_amdgcn_alignbit(sv[2], sv[1], 16),
+            __builtin_amdgcn_alignbit(sv[3], sv[2], 16), __builtin_amdgcn_alignbit(sv[4], sv[3], 16));
+      }
+    }
+  }
+
+  __device__ __forceinline__ static void run(const __half *__restrict__ in, uint64_t ld,
+                                             const int32_t *__restrict__ table,
+                                             const int32_t *__restrict__ reltab, uint32_t nchans,
+                                             uint32_t nsamp, uint32_t ndms, uint32_t wp,
+                                             uint32_t cbr, float *__restrict__ out, uint32_t tt,
+                                             uint32_t dt, float *ldsf) {
+    uint32_t *lds = reinterpret_cast<uint32_t *>(ldsf);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = uniform(tid >> 6);
+    const uint32_t t0 = tt * TT, d0 = dt * TD;
+    const uint32_t dw = d0 + wave * MW;
+    const uint64_t ldw = ld / 2u;  // source dwords per row
+    const uint32_t *src32 = reinterpret_cast<const uint32_t *>(in);
+    uint32_t *raw = lds;               // cbr windows of wp words
+    uint32_t *sums = lds + cbr * wp;   // (cbr / 2) * NS * 2 wp words
+    const uint32_t nblocks = (nchans + cbr - 1u) / cbr;
+    const uint32_t n_dtiles = (ndms + TD - 1u) / TD;
+    const int32_t *dtab =
+        reltab + (uint64_t)n_dtiles * nblocks * 4u * Rel::WORDS + (uint64_t)dt * nblocks * CBP * 2u;
+    const uint32_t lane4 = 4u * lane;
+
+    // window starts of the next block to stage, and its pairs' {dmin, n}
+    uint32_t g0pre[CB];
+    int32_t dminpre[CBP];
+    uint32_t npre[CBP];
+    auto prefetch = [&](uint32_t cb, uint32_t k) {
+#pragma unroll
+      for (int p = 0; p < CBP; ++p) {
+        const uint32_t c0 = cb + 2u * (uint32_t)p;
+        if (2u * (uint32_t)p < cbr && c0 < nchans) {
+          const uint32_t g = t0 + (uint32_t)table[(uint64_t)c0 * ndms + d0];
+          dminpre[p] = dtab[((uint64_t)k * CBP + p) * 2u];
+          npre[p] = (uint32_t)dtab[((uint64_t)k * CBP + p) * 2u + 1u];
+          g0pre[2 * p] = g;
+          g0pre[2 * p + 1] = (g & ~1u) + (uint32_t)dminpre[p];
+        }
+      }
+    };
+    auto stage = [&](uint32_t cb) {
+      const uint32_t nc = min(cbr, nchans - cb);
+#pragma unroll
+      for (int rr = 0; rr < CB; ++rr) {
+        if ((uint32_t)rr >= nc) break;
+        const uint32_t r = (uint32_t)rr;
+        const uint64_t w0 = g0pre[rr] >> 1;
+        const uint32_t *src = src32 + (uint64_t)(cb + r) * ldw + w0;
+        const int32_t ldw32 = (int32_t)ldw;  // rows < 2 GiB
+        const int32_t avail = 4 * (ldw32 - min((int32_t)w0, ldw32));
+        uint32_t *dst = raw + r * wp;
+        for (uint32_t q = wave; 64u * q < wp; q += 4u) {
+          const int32_t rest = max(avail - 256 * (int32_t)q, 0);
+          chunk_to_lds(src + 64u * q, (uint32_t)rest, dst + 64u * q, lane4);
+        }
+      }
+    };
+    int32_t dmin[CBP];
+    uint32_t nrow[CBP];
+    auto take = [&]() {
+#pragma unroll
+      for (int p = 0; p < CBP; ++p) {
+        dmin[p] = dminpre[p];
+        nrow[p] = npre[p];
+      }
+    };
+    // pair p is built by waves p W .. p W + W - 1 (W = 4 / CBP), wave p W + h
+    // taking the 256-word parts h, h + W, ... of the window; all of a pair's
+    // sum rows come from one read of its two windows
+    auto build = [&](uint32_t cb) {
+      const uint32_t nc = min(cbr, nchans - cb);
+      constexpr uint32_t W = 4u / CBP;
+      const uint32_t p = wave / W, h = wave % W;
+      if (2u * p >= nc) return;
+      const bool solo = 2u * p + 1u >= nc;
+      const uint32_t *R0 = raw + 2u * p * wp;
+      uint32_t *S = sums + p * (uint32_t)NS * 2u * wp;
+      const uint32_t k0 = 256u * h + 4u * lane, step = 256u * W;
+      // dmin and nrow of pair p, selected with compile-time indices
+      int32_t dm = dmin[0];
+      uint32_t nr = nrow[0];
+#pragma unroll
+      for (int q = 1; q < CBP; ++q)
+        if (p == (uint32_t)q) dm = dmin[q], nr = nrow[q];
+      if ((uint32_t)dm & 1u) build_pair<1>(R0, R0 + wp, S, wp, nr, solo, k0, step);
+      else build_pair<0>(R0, R0 + wp, S, wp, nr, solo, k0, step);
+    };
+
+    float facc[MW][K2][2];
+#pragma unroll
+    for (int i = 0; i < MW; ++i)
+#pragma unroll
+      for (int j = 0; j < K2; ++j) facc[i][j][0] = facc[i][j][1] = 0.0f;
+    // 1.0 from an opaque scalar move: fma(fpext(h), one, acc) then selects
+    // v_fma_mix_f32 instead of a convert plus an add
+    uint32_t one_bits;
+    asm("s_mov_b32 %0, 0x3f800000" : "=s"(one_bits));
+    const float one = __builtin_bit_cast(float, one_bits);
+
+    Rel rel, rel_next;
+    const int32_t *rt = reltab + ((uint64_t)dt * nblocks * 4u + wave) * Rel::WORDS;
+    prefetch(0, 0);
+    stage(0);
+    take();
+    if (cbr < nchans) prefetch(cbr, 1);
+    rel.load(rt, lane);
+    wait_vmem();
+    __syncthreads();
+    build(0);
+    __syncthreads();
+    const uint32_t *cur = sums + lane;
+    uint32_t k = 0;
+    for (uint32_t cb = 0; cb < nchans; cb += cbr, ++k) {
+      const bool more = cb + cbr < nchans;
+      if (more) {
+        stage(cb + cbr);
+        take();
+        if (cb + 2u * cbr < nchans) prefetch(cb + 2u * cbr, k + 2u);
+        rel_next.load(rt + (uint64_t)(k + 1u) * 4u * Rel::WORDS, lane);
+      }
+      const uint32_t nc = min(cbr, nchans - cb);
+#pragma unroll
+      for (int p = 0; p < CBP; ++p) {
+        if (2u * (uint32_t)p < nc) {
+#pragma unroll
+          for (int i = 0; i < MW; ++i) {
+            const uint32_t *q = at_bytes(cur, rel.get(p, i));
+#pragma unroll
+            for (int j = 0; j < K2; ++j) {
+              const half2_t h = __builtin_bit_cast(half2_t, q[64 * j]);
+              facc[i][j][0] = __builtin_fmaf((float)h.x, one, facc[i][j][0]);
+              facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
+            }
+          }
+        }
+      }
+      wait_vmem();
+      __syncthreads();
+      if (more) {
+        build(cb + cbr);
+        __syncthreads();
+        rel = rel_next;
+      }
+    }
+
+#pragma unroll
+    for (int i = 0; i < MW; ++i) {
+      const uint32_t d = dw + (uint32_t)i;
+      if (d < ndms) {
+        float *o = out + (uint64_t)d * nsamp;
+#pragma unroll
+        for (int j = 0; j < K2; ++j) {
+          const uint32_t t = t0 + 2u * lane + 128u * j;
+          if (t < nsamp) o[t] = facc[i][j][0];
+          if (t + 1u < nsamp) o[t + 1] = facc[i][j][1];
+        }
+      }
+    }
+  }
+};
+
+template <typename T>
+struct IsPairSum : std::false_type {};
+template <int MW, int K2, int CBP, int NS>
+struct IsPairSum<PairSumTile<MW, K2, CBP, NS>> : std::true_type {};
+
+// Trial offsets and {dmin, n} of a pair-sum tile (layout above): entry e <
+// nrel is ((dt * nblocks + k) * 4 + w) * WORDS + idx with (p, i) =
+// divmod(idx, MW); the byte offset of trial dw + i of pair p inside the sum
+// rows, from q0 = D0(d) - D0(d0) + (D0(d0) & 1) (sample position from the
+// even window origin) and j = delta(d) - dmin: word ((p NS + j) 2 + (q0 & 1))
+// wp + q0 / 2.  Entries past nrel hold {dmin, n} per (dt, k, p).  A pair
+// without its second channel (odd nchans) sums with zero: dmin 0, n 1.
+template <typename Tile>
+__global__ __launch_bounds__(256) void pairsum_offset_kernel(const int32_t *__restrict__ table,
+                                                             uint32_t nchans, uint32_t ndms,
+                                                             uint32_t wp, uint32_t cbr,
+                                                             uint32_t nblocks, uint64_t nrel,
+                                                             uint64_t total,
+                                                             int32_t *__restrict__ reltab) {
+  using Rel = typename Tile::Rel;
+  constexpr uint32_t MW = Tile::TD / 4, CBP = Tile::CB / 2, NS = Tile::NSUM;
+  const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (e >= total) return;
+  uint32_t k, dt, p, i = 0, w = 0, field = 0;
+  const bool is_rel = e < nrel;
+  if (is_rel) {
+    const uint32_t idx = (uint32_t)(e % Rel::WORDS);
+    const uint64_t g = e / Rel::WORDS;
+    w = (uint32_t)(g & 3u);
+    const uint64_t blk = g >> 2;
+    k = (uint32_t)(blk % nblocks);
+    dt = (uint32_t)(blk / nblocks);
+    p = min(idx / MW, CBP - 1u);  // padding words repeat the last pair
+    i = idx % MW;
+  } else {
+    const uint64_t q = e - nrel;
+    field = (uint32_t)(q & 1u);
+    const uint64_t pq = q >> 1;
+    p = (uint32_t)(pq % CBP);
+    const uint64_t blk = pq / CBP;
+    k = (uint32_t)(blk % nblocks);
+    dt = (uint32_t)(blk / nblocks);
+  }
+  const uint32_t d0 = dt * Tile::TD, dl = min(d0 + (uint32_t)Tile::TD, ndms) - 1u;
+  const uint32_t c0 = min(k * cbr + 2u * p, nchans - 1u);
+  const bool solo = c0 + 1u >= nchans;
+  const int32_t *r0 = table + (uint64_t)c0 * ndms;
+  const int32_t *r1 = table + (uint64_t)(solo ? c0 : c0 + 1u) * ndms;
+  int32_t dmin = 0, dmax = 0;
+  if (!solo) {
+    dmin = dmax = r1[d0] - r0[d0];
+    for (uint32_t d = d0 + 1u; d <= dl; ++d) {
+      const int32_t v = r1[d] - r0[d];
+      dmin = min(dmin, v);
+      dmax = max(dmax, v);
+    }
+  }
+  if (!is_rel) {
+    reltab[e] = field ? dmax - dmin + 1 : dmin;
+    return;
+  }
+  const uint32_t d = min(d0 + w * MW + i, ndms - 1u);
+  const int32_t base = r0[d0];
+  const uint32_t q0 = (uint32_t)(r0[d] - base) + ((uint32_t)base & 1u);
+  const uint32_t j = solo ? 0u : (uint32_t)(r1[d] - r0[d] - dmin);
+  const uint32_t word = ((p * NS + min(j, NS - 1u)) * 2u + (q0 & 1u)) * wp + (q0 >> 1);
+  reltab[e] = (int32_t)(4u * word);
+}
+
 // XCD-aware workgroup order for the single-beam launch: off unless
 // MDSM_XCD_REMAP=1 (measured at 1024 channels x 4096 trials x 2^20 samples it
 // was level for FP32 and 2-6% slower for FP16/U8 than the plain order)
@@ -799,6 +1111,14 @@ template <> struct VariantDef<14> { using F32 = F32Tile<16, 4, 8>;  using F16 = 
 // SIMD for the packed tiles (68-72 VGPRs)
 template <> struct VariantDef<15> { using F32 = F32Tile<12, 4, 4>;  using F16 = F16Tile<12, 2, 4>;  using U8 = U8Tile<12, 2, 4>; };
 constexpr int kNumVariants = 15;
+// F16_PAIRS only: pair-sum tiles (MW, K2, pairs per block, sum rows per pair)
+template <int V>
+struct PairSumDef;
+template <> struct PairSumDef<16> { using type = PairSumTile<8, 4, 2, 3>; };
+template <> struct PairSumDef<17> { using type = PairSumTile<12, 4, 2, 3>; };
+template <> struct PairSumDef<18> { using type = PairSumTile<12, 2, 2, 3>; };
+template <> struct PairSumDef<19> { using type = PairSumTile<16, 3, 2, 4>; };
+constexpr int kNumPairSumVariants = 4;
 
 template <typename V, typename Fn>
 int pick_dtype(int dtype, Fn &&fn) {
@@ -831,6 +1151,14 @@ int with_tile(int dtype, int variant, Fn &&fn) {
     case 14: return pick_dtype<VariantDef<14>>(dtype, fn);
     case 15: return pick_dtype<VariantDef<15>>(dtype, fn);
   }
+  if (dtype == MDSM_F16_PAIRS) {
+    switch (variant) {
+      case 16: return fn((typename PairSumDef<16>::type *)nullptr);
+      case 17: return fn((typename PairSumDef<17>::type *)nullptr);
+      case 18: return fn((typename PairSumDef<18>::type *)nullptr);
+      case 19: return fn((typename PairSumDef<19>::type *)nullptr);
+    }
+  }
   set_error("unknown dedispersion variant %d", variant);
   return MDSM_ERR_ARG;
 }
@@ -850,7 +1178,9 @@ uint32_t stage_channels(uint32_t rw) {
 
 }  // namespace
 
-int num_variants(int) { return kNumVariants; }
+int num_variants(int dtype) {
+  return dtype == MDSM_F16_PAIRS ? kNumVariants + kNumPairSumVariants : kNumVariants;
+}
 
 int preferred_variant(int dtype) {
   return dtype == MDSM_F32 ? 8 : (storage_dtype(dtype) == MDSM_F16 ? 15 : 2);
@@ -868,11 +1198,35 @@ TileShape variant_shape(int dtype, int variant) {
   return sh;
 }
 
-uint32_t lds_row_words(int dtype, int variant, uint32_t spread) {
+// Pair-sum tiles: every channel pair's delay difference is >= 0 and takes
+// at most ns values inside each DM tile of td trials.
+static bool pairsum_accepts(const int32_t *table, uint32_t nchans, uint32_t ndms, uint32_t td,
+                            uint32_t ns) {
+  for (uint32_t c0 = 0; c0 + 1u < nchans; c0 += 2u) {
+    const int32_t *r0 = table + (uint64_t)c0 * ndms, *r1 = r0 + ndms;
+    for (uint32_t d0 = 0; d0 < ndms; d0 += td) {
+      const uint32_t dl = std::min(d0 + td, ndms);
+      int32_t lo = r1[d0] - r0[d0], hi = lo;
+      for (uint32_t d = d0 + 1u; d < dl; ++d) {
+        const int32_t v = r1[d] - r0[d];
+        lo = std::min(lo, v);
+        hi = std::max(hi, v);
+      }
+      if (lo < 0 || (int64_t)hi - lo + 1 > (int64_t)ns) return false;
+    }
+  }
+  return true;
+}
+
+uint32_t lds_row_words(int dtype, int variant, uint32_t spread, const int32_t *table,
+                       uint32_t nchans, uint32_t ndms) {
   uint32_t rw = 0;
   with_tile(dtype, variant, [&](auto *tp) {
     using Tile = std::remove_pointer_t<decltype(tp)>;
     rw = Tile::fits(spread) ? Tile::row_words(spread) : 0u;
+    if constexpr (IsPairSum<Tile>::value) {
+      if (!pairsum_accepts(table, nchans, ndms, Tile::TD, Tile::NSUM)) rw = 0;
+    }
     return MDSM_OK;
   });
   return rw;
@@ -899,6 +1253,7 @@ uint64_t trial_offset_words(int dtype, int variant, uint32_t nchans, uint32_t nd
     const uint64_t n_dtiles = (ndms + Tile::TD - 1) / Tile::TD;
     const uint64_t nblocks = cbr ? (nchans + cbr - 1) / cbr : 0;
     n = n_dtiles * nblocks * 4u * Tile::Rel::WORDS;
+    if constexpr (IsPairSum<Tile>::value) n += n_dtiles * nblocks * (uint64_t)(Tile::CB / 2) * 2u;
     return MDSM_OK;
   });
   return n;
@@ -913,8 +1268,15 @@ int launch_trial_offsets(int dtype, int variant, const int32_t *table, uint32_t 
     const uint64_t total = trial_offset_words(dtype, variant, nchans, ndms, cbr);
     const uint64_t grid = (total + 255u) / 256u;
     MDSM_REQUIRE(grid < (1ull << 31), MDSM_ERR_ARG, "trial offset table too large");
-    hipLaunchKernelGGL(trial_offset_kernel<Tile>, dim3((uint32_t)grid), dim3(256), 0, s, table,
-                       nchans, ndms, rw, cbr, nblocks, total, reltab);
+    if constexpr (IsPairSum<Tile>::value) {
+      MDSM_REQUIRE(cbr % 2u == 0u, MDSM_ERR_ARG, "pair-sum tiles stage whole channel pairs");
+      const uint64_t nrel = (uint64_t)((ndms + Tile::TD - 1) / Tile::TD) * nblocks * 4u * Tile::Rel::WORDS;
+      hipLaunchKernelGGL(pairsum_offset_kernel<Tile>, dim3((uint32_t)grid), dim3(256), 0, s, table,
+                         nchans, ndms, rw, cbr, nblocks, nrel, total, reltab);
+    } else {
+      hipLaunchKernelGGL(trial_offset_kernel<Tile>, dim3((uint32_t)grid), dim3(256), 0, s, table,
+                         nchans, ndms, rw, cbr, nblocks, total, reltab);
+    }
     MDSM_HIP(hipGetLastError());
     return MDSM_OK;
   });

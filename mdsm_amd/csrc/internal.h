@@ -59,8 +59,10 @@ TileShape variant_shape(int dtype, int variant);
 uint32_t tile_spread(const int32_t *table, uint32_t nchans, uint32_t ndms, int td);
 
 // LDS words per staged channel row of a variant for a delay spread; 0 when
-// the variant cannot stage that spread.
-uint32_t lds_row_words(int dtype, int variant, uint32_t spread);
+// the variant cannot stage that spread or (pair-sum tiles) cannot take the
+// survey's delay table (nchans x ndms).
+uint32_t lds_row_words(int dtype, int variant, uint32_t spread, const int32_t *table,
+                       uint32_t nchans, uint32_t ndms);
 // Required multiple (in elements) of the input row length: 1 (F32), 2 (F16, F16_PAIRS), 4 (U8).
 uint32_t input_alignment(int dtype);
 

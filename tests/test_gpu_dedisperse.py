@@ -133,9 +133,48 @@ def test_packed_ragged_shapes_all_variants(shape):
             assert p16p.variant_channels(v) % 2 == 0, v
             assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref16p), ("f16 pairs", v)
         assert np.array_equal(abi.dedisperse_host(s, u, abi.U8, v), ref8), ("u8", v)
+    # pair-sum tiles: equal where they take the survey, a clean error where not
+    for v in range(abi.tile_variants() + 1, abi.tile_variants(abi.F16_PAIRS) + 1):
+        if p16p.variant_channels(v) == 0:
+            with pytest.raises(abi.MdsmError):
+                abi.dedisperse_host(s, h, abi.F16_PAIRS, v)
+        else:
+            assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref16p), ("pair sums", v)
 
 
-@pytest.mark.parametrize("C", [2, 3, 10])
+# (nchans, ndms, nsamp, lowdm, dmstep, foff): surveys whose channel pairs
+# differ in delay by at most three values per DM tile, so the pair-sum tile
+# shapes take them (odd channel counts, partial DM tiles and time tiles)
+PAIRSUM_SURVEYS = [(256, 100, 1300, 0.0, 0.1, -0.078125), (255, 70, 1000, 0.0, 0.1, -0.078125),
+                   (128, 64, 2100, 5.0, 0.1, -0.078125), (99, 49, 777, 0.0, 0.1, -0.078125),
+                   (512, 40, 600, 0.0, 0.25, -0.0390625), (1024, 300, 5000, 0.0, 0.2, -0.0390625)]
+
+
+@pytest.mark.parametrize("shape", PAIRSUM_SURVEYS)
+def test_pairsum_tiles_bitexact(shape):
+    """The pair-sum tile shapes (each pair's FP16 sums built once per delay
+    difference, then read by the trials) equal the oracle's channel-pair
+    sums bit for bit, like the F16_PAIRS forms of the F16 shapes."""
+    C, M, N, lo, step, foff = shape
+    s = _survey(N, nchans=C, ndms=M, lowdm=lo, dmstep=step, foff=foff)
+    t = _table(s)
+    S = int(t.max())
+    _say(f"[pair-sum tiles] C={C} M={M} N={N} S={S}")
+    rng = np.random.default_rng(C + M)
+    h = O.f32_to_f16((rng.standard_normal((C, N + S + 6)) * 3).astype(np.float32))
+    ref = O.dedisperse_f16_pairs(h, t, N)
+    p = abi.Plan(s, abi.F16_PAIRS)
+    taken = 0
+    for v in range(abi.tile_variants() + 1, abi.tile_variants(abi.F16_PAIRS) + 1):
+        if p.variant_channels(v) == 0:
+            continue
+        taken += 1
+        assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref), v
+    # the first (narrowest) shape takes every survey of this list
+    assert taken >= 1 and p.variant_channels(abi.tile_variants() + 1) > 0
+
+
+@pytest.mark.parametrize("C", [2, 3, 10, 65])
 def test_f16_pairs_rounding_cases_all_variants(C):
     """The FP16 pair sums at their rounding edges: ties to even (2048 + 1,
     1 + 2^-11, 1 + 3*2^-11), the overflow tie (65504 + 16 -> inf, and its
@@ -158,10 +197,12 @@ def test_f16_pairs_rounding_cases_all_variants(C):
     ref = O.dedisperse_f16_pairs(h, t, N)
     _say(f"[f16 pairs edges] C={C} M={M} N={N} S={S}")
     p = abi.Plan(s, abi.F16_PAIRS)
-    for v in range(1, abi.tile_variants() + 1):
+    for v in range(1, abi.tile_variants(abi.F16_PAIRS) + 1):
         if p.variant_channels(v) == 0:
             continue
         assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref), v
+    if C >= 10:  # the pair-sum shapes take these (adjacent delays differ by <= 1 over 3 trials)
+        assert p.variant_channels(abi.tile_variants() + 1) > 0
 
 
 def test_u8_near_saturated_sums_all_variants():
