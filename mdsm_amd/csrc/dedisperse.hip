@@ -29,6 +29,9 @@
 //        FP16 rounding) and the pair sum goes into the FP32 accumulators
 //        with v_fma_mix_f32: three VALU instructions per four adds instead
 //        of four, at about 1.7x the rounding error of F16 against FP64.
+//        The pair-sum tiles build those sums once per delay difference of
+//        the two channels instead and read one word per trial (one LDS word
+//        and two v_fma_mix_f32 per four adds), where the survey allows.
 //   U8   2 B per add: bytes widened to pairs of u16 in the same two-copy
 //        layout, summed exactly with v_pk_add_u16 (two adds per
 //        instruction); every <= 256 channels their high bytes move into a

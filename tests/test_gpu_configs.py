@@ -115,11 +115,17 @@ def _channel_constant_expect(dtype, C):
     return np.float32(v.sum())
 
 
-@pytest.mark.parametrize("dtype", [abi.F32, abi.F16, abi.U8, abi.F16_PAIRS],
-                         ids=["f32", "f16", "u8", "f16_pairs"])
-def test_batch_32_beams_full_size(batch32, dtype):
+@pytest.mark.parametrize("dtype,variant", [(abi.F32, None), (abi.F16, None), (abi.U8, None),
+                                           (abi.F16_PAIRS, None), (abi.F16_PAIRS, 17)],
+                         ids=["f32", "f16", "u8", "f16_pairs", "f16_pairsum_tile"])
+def test_batch_32_beams_full_size(batch32, dtype, variant):
+    """The 32-beam batch in each storage type (untuned shape; the channel
+    pair sums also with the 12 x 4 pair-sum tile shape, which takes every
+    beam of this batch)."""
     beams, surveys, outs, checked = batch32
     batch = abi.Batch(surveys, dtype)
+    if variant is not None:
+        batch.set_variant(variant)
     isz = ISZ[dtype]
     st = bench.Stage(None)
     ins = []
@@ -130,7 +136,8 @@ def test_batch_32_beams_full_size(batch32, dtype):
         ins.append((d, ld))
     abi.synchronize()
     args = ([d.ptr for d, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
-    log(f"[32-beam batch {NAME[dtype]}] {len(beams)} beams generated and normalised on the device")
+    log(f"[32-beam batch {NAME[dtype]}] {len(beams)} beams generated and normalised on the device, "
+        f"tile shape {variant or 'untuned'}")
 
     sampled = {}
     for mode in (abi.BATCH_PERSISTENT, abi.BATCH_FLAT):
