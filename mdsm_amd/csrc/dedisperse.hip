@@ -639,23 +639,83 @@ struct PairsOf<PackedTile<MDSM_F16, MW, K2, CB, BURST>> {
 // the sum rows, so the next block's windows can arrive while it runs.
 // Table: the trial offsets (one per pair and trial, relative to the sum
 // rows), then per (DM tile, block, pair) the words {dmin, n}.
-template <int MW, int K2, int CBP, int NS>
+template <int MODE, int MW, int K2, int CBP, int NS>
 struct PairSumTile {
   static constexpr int TT = 128 * K2;
   static constexpr int TD = 4 * MW;
   static constexpr int CB = 2 * CBP;
   static constexpr bool PAIRS = true;
+  static constexpr bool IS_U8 = MODE == MDSM_U8;
   static constexpr int NSUM = NS;
+  // window origins on even samples (F16: one dword = 2 samples) or on
+  // multiples of 4 (U8: one dword = 4 samples)
+  static constexpr uint32_t ORIGIN = IS_U8 ? 4u : 2u;
   static constexpr int PACK = 1;
-  using T = __half;
+  using T = std::conditional_t<IS_U8, uint8_t, __half>;
   using Rel = RelOffsets<MW, CBP, PACK>;
+  static_assert(MODE == MDSM_F16_PAIRS || MODE == MDSM_U8, "pair sums of F16 or U8 rows");
   static_assert(NS >= 1 && NS <= 4, "sum rows per pair (shifts up to 4 samples)");
   static_assert(CBP == 1 || CBP == 2 || CBP == 4, "waves per pair in the build: 4 / CBP");
   static bool fits(uint32_t) { return true; }
   // rw: words of one copy (window of TT + spread samples from an even origin)
   static uint32_t row_words(uint32_t spread) { return PackedGeom<MW, K2, CB>::copy_words(spread); }
+  // raw window dwords per channel: F16 one copy; U8 the bytes of 2 rw + 8
+  // samples, padded to 64
+  static __host__ __device__ uint32_t raw_words(uint32_t rw) {
+    return IS_U8 ? ((rw / 2u + 2u + 63u) & ~63u) : rw;
+  }
   // per staged channel: its raw window and half of its pair's sum rows
-  static uint32_t lds_words_per_row(uint32_t rw) { return (1u + (uint32_t)NS) * rw; }
+  static uint32_t lds_words_per_row(uint32_t rw) { return raw_words(rw) + (uint32_t)NS * rw; }
+
+  // U8: the n sum rows of one pair, exact u16 pairs: row j is P_j[k] =
+  // (x0[2k] + x1[2k + SH], x0[2k + 1] + x1[2k + 1 + SH]) with SH = j + F,
+  // F = dmin & 3, samples counted from the 4-aligned origins of the two raw
+  // windows (bytes widened by v_perm_b32, added by v_pk_add_u16; sums <=
+  // 510), and its Q copy.  Lane l builds words k0 + 4 l .. + 3 of every
+  // `step` from 3 dwords of R0 and 4 of R1 (reads end at most 2 dwords past
+  // word wp / 2 of a row, inside the row's padding).
+  template <int F>
+  __device__ __forceinline__ static void build_pair_u8(const uint32_t *R0, const uint32_t *R1,
+                                                       uint32_t *S, uint32_t wp, uint32_t n,
+                                                       bool solo, uint32_t k0, uint32_t step) {
+    for (uint32_t k = k0; k < wp; k += step) {
+      const uint2 a01 = *reinterpret_cast<const uint2 *>(R0 + k / 2u);
+      const uint32_t a2 = R0[k / 2u + 2u];
+      // x0 samples 2i, 2i + 1 as a u16 pair, i = 0..4
+      const uint32_t av[5] = {__builtin_amdgcn_perm(0u, a01.x, 0x0C010C00u),
+                              __builtin_amdgcn_perm(0u, a01.x, 0x0C030C02u),
+                              __builtin_amdgcn_perm(0u, a01.y, 0x0C010C00u),
+                              __builtin_amdgcn_perm(0u, a01.y, 0x0C030C02u),
+                              __builtin_amdgcn_perm(0u, a2, 0x0C010C00u)};
+      uint32_t z[4] = {0u, 0u, 0u, 0u};
+      if (!solo) {
+        const uint2 z01 = *reinterpret_cast<const uint2 *>(R1 + k / 2u);
+        const uint2 z23 = *reinterpret_cast<const uint2 *>(R1 + k / 2u + 2u);
+        z[0] = z01.x, z[1] = z01.y, z[2] = z23.x, z[3] = z23.y;
+      }
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        if ((uint32_t)j >= n) break;
+        const int sh = j + F;
+        uint32_t sv[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+          // x1 bytes b = 2i + sh and b + 1: dword b / 4 (and the next one
+          // when b % 4 == 3), selected into the low bytes of the two halves
+          const int bb = 2 * i + sh, w = bb >> 2, o = bb & 3;
+          const uint32_t lo = z[w], hi = o == 3 ? z[w + 1] : z[w];
+          const uint32_t bv = __builtin_amdgcn_perm(hi, lo, 0x0C000C00u | (uint32_t)o | ((uint32_t)(o + 1) << 16));
+          const ushort2_t sum = __builtin_bit_cast(ushort2_t, av[i]) + __builtin_bit_cast(ushort2_t, bv);
+          sv[i] = __builtin_bit_cast(uint32_t, sum);
+        }
+        uint32_t *P = S + (uint32_t)j * 2u * wp;
+        *reinterpret_cast<uint4 *>(P + k) = make_uint4(sv[0], sv[1], sv[2], sv[3]);
+        *reinterpret_cast<uint4 *>(P + wp + k) = make_uint4(
+            __builtin_amdgcn_alignbit(sv[1], sv[0], 16), __builtin_amdgcn_alignbit(sv[2], sv[1], 16),
+            __builtin_amdgcn_alignbit(sv[3], sv[2], 16), __builtin_amdgcn_alignbit(sv[4], sv[3], 16));
+      }
+    }
+  }
 
   // The n sum rows of one pair over part of the window: row j is
   // P_j[k] = fl16(R0[k] + R1 shifted by SH = j + F samples) (two halves per
@@ -699,7 +759,7 @@ struct PairSumTile {
     }
   }
 
-  __device__ __forceinline__ static void run(const __half *__restrict__ in, uint64_t ld,
+  __device__ __forceinline__ static void run(const T *__restrict__ in, uint64_t ld,
                                              const int32_t *__restrict__ table,
                                              const int32_t *__restrict__ reltab, uint32_t nchans,
                                              uint32_t nsamp, uint32_t ndms, uint32_t wp,
@@ -711,10 +771,11 @@ struct PairSumTile {
     const uint32_t wave = uniform(tid >> 6);
     const uint32_t t0 = tt * TT, d0 = dt * TD;
     const uint32_t dw = d0 + wave * MW;
-    const uint64_t ldw = ld / 2u;  // source dwords per row
+    const uint64_t ldw = IS_U8 ? ld / 4u : ld / 2u;  // source dwords per row
     const uint32_t *src32 = reinterpret_cast<const uint32_t *>(in);
-    uint32_t *raw = lds;               // cbr windows of wp words
-    uint32_t *sums = lds + cbr * wp;   // (cbr / 2) * NS * 2 wp words
+    const uint32_t rawn = raw_words(wp);
+    uint32_t *raw = lds;                 // cbr windows of rawn dwords
+    uint32_t *sums = lds + cbr * rawn;   // (cbr / 2) * NS * 2 wp words
     const uint32_t nblocks = (nchans + cbr - 1u) / cbr;
     const uint32_t n_dtiles = (ndms + TD - 1u) / TD;
     const int32_t *dtab =
@@ -734,7 +795,7 @@ struct PairSumTile {
           dminpre[p] = dtab[((uint64_t)k * CBP + p) * 2u];
           npre[p] = (uint32_t)dtab[((uint64_t)k * CBP + p) * 2u + 1u];
           g0pre[2 * p] = g;
-          g0pre[2 * p + 1] = (g & ~1u) + (uint32_t)dminpre[p];
+          g0pre[2 * p + 1] = (g & ~(ORIGIN - 1u)) + (uint32_t)dminpre[p];
         }
       }
     };
@@ -744,12 +805,12 @@ struct PairSumTile {
       for (int rr = 0; rr < CB; ++rr) {
         if ((uint32_t)rr >= nc) break;
         const uint32_t r = (uint32_t)rr;
-        const uint64_t w0 = g0pre[rr] >> 1;
+        const uint64_t w0 = IS_U8 ? g0pre[rr] >> 2 : g0pre[rr] >> 1;
         const uint32_t *src = src32 + (uint64_t)(cb + r) * ldw + w0;
         const int32_t ldw32 = (int32_t)ldw;  // rows < 2 GiB
         const int32_t avail = 4 * (ldw32 - min((int32_t)w0, ldw32));
-        uint32_t *dst = raw + r * wp;
-        for (uint32_t q = wave; 64u * q < wp; q += 4u) {
+        uint32_t *dst = raw + r * rawn;
+        for (uint32_t q = wave; 64u * q < rawn; q += 4u) {
           const int32_t rest = max(avail - 256 * (int32_t)q, 0);
           chunk_to_lds(src + 64u * q, (uint32_t)rest, dst + 64u * q, lane4);
         }
@@ -773,7 +834,7 @@ struct PairSumTile {
       const uint32_t p = wave / W, h = wave % W;
       if (2u * p >= nc) return;
       const bool solo = 2u * p + 1u >= nc;
-      const uint32_t *R0 = raw + 2u * p * wp;
+      const uint32_t *R0 = raw + 2u * p * rawn;
       uint32_t *S = sums + p * (uint32_t)NS * 2u * wp;
       const uint32_t k0 = 256u * h + 4u * lane, step = 256u * W;
       // dmin and nrow of pair p, selected with compile-time indices
@@ -782,15 +843,37 @@ struct PairSumTile {
 #pragma unroll
       for (int q = 1; q < CBP; ++q)
         if (p == (uint32_t)q) dm = dmin[q], nr = nrow[q];
-      if ((uint32_t)dm & 1u) build_pair<1>(R0, R0 + wp, S, wp, nr, solo, k0, step);
-      else build_pair<0>(R0, R0 + wp, S, wp, nr, solo, k0, step);
+      if constexpr (IS_U8) {
+        switch ((uint32_t)dm & 3u) {
+          case 0: build_pair_u8<0>(R0, R0 + rawn, S, wp, nr, solo, k0, step); break;
+          case 1: build_pair_u8<1>(R0, R0 + rawn, S, wp, nr, solo, k0, step); break;
+          case 2: build_pair_u8<2>(R0, R0 + rawn, S, wp, nr, solo, k0, step); break;
+          default: build_pair_u8<3>(R0, R0 + rawn, S, wp, nr, solo, k0, step); break;
+        }
+      } else {
+        if ((uint32_t)dm & 1u) build_pair<1>(R0, R0 + rawn, S, wp, nr, solo, k0, step);
+        else build_pair<0>(R0, R0 + rawn, S, wp, nr, solo, k0, step);
+      }
     };
 
-    float facc[MW][K2][2];
+    // F16: FP32 accumulators.  U8: u16 pair accumulators with the running
+    // sum S = 256 * uhi + uacc held exactly, as in the U8 tile: a fold moves
+    // uacc's high bytes into uhi and leaves uacc <= 255, so 256 more channels
+    // (128 pair sums <= 510) fit in 16 bits.
+    float facc[IS_U8 ? 1 : MW][IS_U8 ? 1 : K2][2];
+    ushort2_t uacc[IS_U8 ? MW : 1][IS_U8 ? K2 : 1];
+    ushort2_t uhi[IS_U8 ? MW : 1][IS_U8 ? K2 : 1];
 #pragma unroll
     for (int i = 0; i < MW; ++i)
 #pragma unroll
-      for (int j = 0; j < K2; ++j) facc[i][j][0] = facc[i][j][1] = 0.0f;
+      for (int j = 0; j < K2; ++j) {
+        if constexpr (IS_U8) {
+          uacc[i][j] = ushort2_t{0, 0};
+          uhi[i][j] = ushort2_t{0, 0};
+        } else {
+          facc[i][j][0] = facc[i][j][1] = 0.0f;
+        }
+      }
     // 1.0 from an opaque scalar move: fma(fpext(h), one, acc) then selects
     // v_fma_mix_f32 instead of a convert plus an add
     uint32_t one_bits;
@@ -809,7 +892,7 @@ struct PairSumTile {
     build(0);
     __syncthreads();
     const uint32_t *cur = sums + lane;
-    uint32_t k = 0;
+    uint32_t k = 0, since_fold = 0;
     for (uint32_t cb = 0; cb < nchans; cb += cbr, ++k) {
       const bool more = cb + cbr < nchans;
       if (more) {
@@ -819,6 +902,19 @@ struct PairSumTile {
         rel_next.load(rt + (uint64_t)(k + 1u) * 4u * Rel::WORDS, lane);
       }
       const uint32_t nc = min(cbr, nchans - cb);
+      if constexpr (IS_U8) {
+        if (since_fold + nc > 256u) {  // 255 + 128 * 510 = 65535 still fits in 16 bits
+#pragma unroll
+          for (int i = 0; i < MW; ++i)
+#pragma unroll
+            for (int j = 0; j < K2; ++j) {
+              uhi[i][j] += uacc[i][j] >> (ushort)8;
+              uacc[i][j] &= (ushort)0xFF;
+            }
+          since_fold = 0;
+        }
+        since_fold += nc;
+      }
 #pragma unroll
       for (int p = 0; p < CBP; ++p) {
         if (2u * (uint32_t)p < nc) {
@@ -827,9 +923,13 @@ struct PairSumTile {
             const uint32_t *q = at_bytes(cur, rel.get(p, i));
 #pragma unroll
             for (int j = 0; j < K2; ++j) {
-              const half2_t h = __builtin_bit_cast(half2_t, q[64 * j]);
-              facc[i][j][0] = __builtin_fmaf((float)h.x, one, facc[i][j][0]);
-              facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
+              if constexpr (IS_U8) {
+                uacc[i][j] += __builtin_bit_cast(ushort2_t, q[64 * j]);
+              } else {
+                const half2_t h = __builtin_bit_cast(half2_t, q[64 * j]);
+                facc[i][j][0] = __builtin_fmaf((float)h.x, one, facc[i][j][0]);
+                facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
+              }
             }
           }
         }
@@ -843,6 +943,7 @@ struct PairSumTile {
       }
     }
 
+    const double base = (double)nchans * -4.0;
 #pragma unroll
     for (int i = 0; i < MW; ++i) {
       const uint32_t d = dw + (uint32_t)i;
@@ -851,8 +952,18 @@ struct PairSumTile {
 #pragma unroll
         for (int j = 0; j < K2; ++j) {
           const uint32_t t = t0 + 2u * lane + 128u * j;
-          if (t < nsamp) o[t] = facc[i][j][0];
-          if (t + 1u < nsamp) o[t + 1] = facc[i][j][1];
+          float v0, v1;
+          if constexpr (IS_U8) {
+            const uint32_t s0 = 256u * (uint32_t)uhi[i][j].x + (uint32_t)uacc[i][j].x;
+            const uint32_t s1 = 256u * (uint32_t)uhi[i][j].y + (uint32_t)uacc[i][j].y;
+            v0 = (float)((double)s0 * (8.0 / 255.0) + base);
+            v1 = (float)((double)s1 * (8.0 / 255.0) + base);
+          } else {
+            v0 = facc[i][j][0];
+            v1 = facc[i][j][1];
+          }
+          if (t < nsamp) o[t] = v0;
+          if (t + 1u < nsamp) o[t + 1] = v1;
         }
       }
     }
@@ -861,8 +972,8 @@ struct PairSumTile {
 
 template <typename T>
 struct IsPairSum : std::false_type {};
-template <int MW, int K2, int CBP, int NS>
-struct IsPairSum<PairSumTile<MW, K2, CBP, NS>> : std::true_type {};
+template <int MODE, int MW, int K2, int CBP, int NS>
+struct IsPairSum<PairSumTile<MODE, MW, K2, CBP, NS>> : std::true_type {};
 
 // Trial offsets and {dmin, n} of a pair-sum tile (layout above): entry e <
 // nrel is ((dt * nblocks + k) * 4 + w) * WORDS + idx with (p, i) =
@@ -922,7 +1033,7 @@ __global__ __launch_bounds__(256) void pairsum_offset_kernel(const int32_t *__re
   }
   const uint32_t d = min(d0 + w * MW + i, ndms - 1u);
   const int32_t base = r0[d0];
-  const uint32_t q0 = (uint32_t)(r0[d] - base) + ((uint32_t)base & 1u);
+  const uint32_t q0 = (uint32_t)(r0[d] - base) + ((uint32_t)base & (Tile::ORIGIN - 1u));
   const uint32_t j = solo ? 0u : (uint32_t)(r1[d] - r0[d] - dmin);
   const uint32_t word = ((p * NS + min(j, NS - 1u)) * 2u + (q0 & 1u)) * wp + (q0 >> 1);
   reltab[e] = (int32_t)(4u * word);
@@ -1114,13 +1225,13 @@ template <> struct VariantDef<14> { using F32 = F32Tile<16, 4, 8>;  using F16 = 
 // SIMD for the packed tiles (68-72 VGPRs)
 template <> struct VariantDef<15> { using F32 = F32Tile<12, 4, 4>;  using F16 = F16Tile<12, 2, 4>;  using U8 = U8Tile<12, 2, 4>; };
 constexpr int kNumVariants = 15;
-// F16_PAIRS only: pair-sum tiles (MW, K2, pairs per block, sum rows per pair)
+// F16_PAIRS and U8 only: pair-sum tiles (MW, K2, pairs per block, sum rows per pair)
 template <int V>
 struct PairSumDef;
-template <> struct PairSumDef<16> { using type = PairSumTile<8, 4, 2, 3>; };
-template <> struct PairSumDef<17> { using type = PairSumTile<12, 4, 2, 3>; };
-template <> struct PairSumDef<18> { using type = PairSumTile<12, 2, 2, 3>; };
-template <> struct PairSumDef<19> { using type = PairSumTile<16, 3, 2, 4>; };
+template <> struct PairSumDef<16> { template <int M> using type = PairSumTile<M, 8, 4, 2, 3>; };
+template <> struct PairSumDef<17> { template <int M> using type = PairSumTile<M, 12, 4, 2, 3>; };
+template <> struct PairSumDef<18> { template <int M> using type = PairSumTile<M, 12, 2, 2, 3>; };
+template <> struct PairSumDef<19> { template <int M> using type = PairSumTile<M, 16, 3, 2, 4>; };
 constexpr int kNumPairSumVariants = 4;
 
 template <typename V, typename Fn>
@@ -1156,10 +1267,18 @@ int with_tile(int dtype, int variant, Fn &&fn) {
   }
   if (dtype == MDSM_F16_PAIRS) {
     switch (variant) {
-      case 16: return fn((typename PairSumDef<16>::type *)nullptr);
-      case 17: return fn((typename PairSumDef<17>::type *)nullptr);
-      case 18: return fn((typename PairSumDef<18>::type *)nullptr);
-      case 19: return fn((typename PairSumDef<19>::type *)nullptr);
+      case 16: return fn((PairSumDef<16>::type<MDSM_F16_PAIRS> *)nullptr);
+      case 17: return fn((PairSumDef<17>::type<MDSM_F16_PAIRS> *)nullptr);
+      case 18: return fn((PairSumDef<18>::type<MDSM_F16_PAIRS> *)nullptr);
+      case 19: return fn((PairSumDef<19>::type<MDSM_F16_PAIRS> *)nullptr);
+    }
+  }
+  if (dtype == MDSM_U8) {
+    switch (variant) {
+      case 16: return fn((PairSumDef<16>::type<MDSM_U8> *)nullptr);
+      case 17: return fn((PairSumDef<17>::type<MDSM_U8> *)nullptr);
+      case 18: return fn((PairSumDef<18>::type<MDSM_U8> *)nullptr);
+      case 19: return fn((PairSumDef<19>::type<MDSM_U8> *)nullptr);
     }
   }
   set_error("unknown dedispersion variant %d", variant);
@@ -1182,7 +1301,8 @@ uint32_t stage_channels(uint32_t rw) {
 }  // namespace
 
 int num_variants(int dtype) {
-  return dtype == MDSM_F16_PAIRS ? kNumVariants + kNumPairSumVariants : kNumVariants;
+  return (dtype == MDSM_F16_PAIRS || dtype == MDSM_U8) ? kNumVariants + kNumPairSumVariants
+                                                       : kNumVariants;
 }
 
 int preferred_variant(int dtype) {

@@ -74,11 +74,12 @@ def test_batch_uneven_beams_bitexact(dtype):
                 assert np.array_equal(got, refs[b]), (mode, v, b)
 
 
-def test_batch_pairsum_tiles_bitexact():
-    """F16 channel-pair sums over an uneven batch of narrow-channel beams
-    (odd channel counts among them) that the pair-sum tile shapes take:
-    every such shape, in both launch forms, and the autotuner's pick equal
-    the oracle bit for bit."""
+@pytest.mark.parametrize("dtype", [abi.F16_PAIRS, abi.U8])
+def test_batch_pairsum_tiles_bitexact(dtype):
+    """F16 channel-pair sums and U8 over an uneven batch of narrow-channel
+    beams (odd channel counts among them) that the pair-sum tile shapes
+    take: every such shape, in both launch forms, and the autotuner's pick
+    equal the oracle bit for bit."""
     rng = np.random.default_rng(515)
     beams = []
     for k in range(7):
@@ -87,21 +88,25 @@ def test_batch_pairsum_tiles_bitexact():
         M = int(rng.integers(30, 200))
         beams.append(abi.Survey.make(418.0, -0.078125 * 256 / C, C, 64e-6, 0.0,
                                      float(rng.uniform(0.05, 0.2)), M, N))
-    batch = abi.Batch(beams, abi.F16_PAIRS)
-    print(f"[batch pair sums] {len(beams)} beams", flush=True, file=sys.stderr)
+    batch = abi.Batch(beams, dtype)
+    print(f"[batch pair sums] dtype {dtype}, {len(beams)} beams", flush=True, file=sys.stderr)
     ins, outs, refs, keep = [], [], [], []
     for b, s in enumerate(beams):
-        ld = batch.min_ld[b] + 2
-        h = O.f32_to_f16(rng.standard_normal((s.nchans, ld)).astype(np.float32))
+        ld = batch.min_ld[b] + 4
         t, _ = abi.delay_table(s)
-        refs.append(O.dedisperse_f16_pairs(h, t, s.nsamp))
+        if dtype == abi.U8:
+            h = rng.integers(0, 256, (s.nchans, ld)).astype(np.uint8)
+            refs.append(O.dedisperse_u8(h, t, s.nsamp))
+        else:
+            h = O.f32_to_f16(rng.standard_normal((s.nchans, ld)).astype(np.float32))
+            refs.append(O.dedisperse_f16_pairs(h, t, s.nsamp))
         di = abi.DeviceBuffer.from_numpy(h)
         do = abi.DeviceBuffer(s.ndms * s.nsamp * 4)
         keep += [di, do]
         ins.append((di.ptr, ld))
         outs.append(do)
     args = ([p for p, _ in ins], [l for _, l in ins], [o.ptr for o in outs])
-    shapes = list(range(abi.tile_variants() + 1, abi.tile_variants(abi.F16_PAIRS) + 1))
+    shapes = list(range(abi.tile_variants() + 1, abi.tile_variants(dtype) + 1))
     for mode in (abi.BATCH_PERSISTENT, abi.BATCH_FLAT):
         batch.set_mode(mode)
         for v in shapes + ["tuned"]:
@@ -110,7 +115,7 @@ def test_batch_pairsum_tiles_bitexact():
             print(f"[batch pair sums] mode {mode} variant {v}", flush=True, file=sys.stderr)
             if v == "tuned":
                 got_v, ms = batch.autotune(*args)
-                assert 1 <= got_v <= abi.tile_variants(abi.F16_PAIRS) and ms > 0
+                assert 1 <= got_v <= abi.tile_variants(dtype) and ms > 0
             else:
                 batch.set_variant(v)
             try:

@@ -133,13 +133,16 @@ def test_packed_ragged_shapes_all_variants(shape):
             assert p16p.variant_channels(v) % 2 == 0, v
             assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref16p), ("f16 pairs", v)
         assert np.array_equal(abi.dedisperse_host(s, u, abi.U8, v), ref8), ("u8", v)
-    # pair-sum tiles: equal where they take the survey, a clean error where not
+    # pair-sum tiles (F16 pair sums, U8): equal where they take the survey,
+    # a clean error where not
+    p8 = abi.Plan(s, abi.U8)
     for v in range(abi.tile_variants() + 1, abi.tile_variants(abi.F16_PAIRS) + 1):
-        if p16p.variant_channels(v) == 0:
-            with pytest.raises(abi.MdsmError):
-                abi.dedisperse_host(s, h, abi.F16_PAIRS, v)
-        else:
-            assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref16p), ("pair sums", v)
+        for plan, dt, arr, ref in ((p16p, abi.F16_PAIRS, h, ref16p), (p8, abi.U8, u, ref8)):
+            if plan.variant_channels(v) == 0:
+                with pytest.raises(abi.MdsmError):
+                    abi.dedisperse_host(s, arr, dt, v)
+            else:
+                assert np.array_equal(abi.dedisperse_host(s, arr, dt, v), ref), ("pair-sum tile", dt, v)
 
 
 # (nchans, ndms, nsamp, lowdm, dmstep, foff): surveys whose channel pairs
@@ -162,16 +165,16 @@ def test_pairsum_tiles_bitexact(shape):
     _say(f"[pair-sum tiles] C={C} M={M} N={N} S={S}")
     rng = np.random.default_rng(C + M)
     h = O.f32_to_f16((rng.standard_normal((C, N + S + 6)) * 3).astype(np.float32))
-    ref = O.dedisperse_f16_pairs(h, t, N)
-    p = abi.Plan(s, abi.F16_PAIRS)
-    taken = 0
-    for v in range(abi.tile_variants() + 1, abi.tile_variants(abi.F16_PAIRS) + 1):
-        if p.variant_channels(v) == 0:
-            continue
-        taken += 1
-        assert np.array_equal(abi.dedisperse_host(s, h, abi.F16_PAIRS, v), ref), v
-    # the first (narrowest) shape takes every survey of this list
-    assert taken >= 1 and p.variant_channels(abi.tile_variants() + 1) > 0
+    u = rng.integers(0, 256, (C, N + S + 8)).astype(np.uint8)
+    for dt, arr, ref in ((abi.F16_PAIRS, h, O.dedisperse_f16_pairs(h, t, N)),
+                         (abi.U8, u, O.dedisperse_u8(u, t, N))):
+        p = abi.Plan(s, dt)
+        for v in range(abi.tile_variants() + 1, abi.tile_variants(dt) + 1):
+            if p.variant_channels(v) == 0:
+                continue
+            assert np.array_equal(abi.dedisperse_host(s, arr, dt, v), ref), (dt, v)
+        # the first (narrowest) shape takes every survey of this list
+        assert p.variant_channels(abi.tile_variants() + 1) > 0
 
 
 @pytest.mark.parametrize("C", [2, 3, 10, 65])
@@ -216,7 +219,10 @@ def test_u8_near_saturated_sums_all_variants():
     u = np.random.default_rng(31).integers(250, 256, (C, N + S + 4)).astype(np.uint8)
     u[:, ::7] = 255
     ref8 = O.dedisperse_u8(u, t, N)
-    for v in range(1, abi.tile_variants() + 1):
+    p8 = abi.Plan(s, abi.U8)
+    for v in range(1, abi.tile_variants(abi.U8) + 1):
+        if v > abi.tile_variants():  # pair-sum tiles: pair sums up to 510, folded every 128 pairs
+            assert p8.variant_channels(v) > 0, v
         assert np.array_equal(abi.dedisperse_host(s, u, abi.U8, v), ref8), ("u8", v)
 
 
