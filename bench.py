@@ -470,6 +470,8 @@ def main():
     ap.add_argument("--profile", action="store_true", help="only the batched FP16 launches")
     ap.add_argument("--profile-pairs", action="store_true",
                     help="with --profile: the batched launches with FP16 channel-pair sums")
+    ap.add_argument("--profile-u8", action="store_true",
+                    help="with --profile: the batched launches with U8 storage")
     ap.add_argument("--variant", type=int, default=None,
                     help="tile variant of the batched FP16 launch instead of the autotuner's "
                          "(for profiling one tile shape)")
@@ -608,9 +610,13 @@ def main():
         return batch, ins, ms, B, v
 
     # the headline storage (in --profile runs optionally the channel-pair sums)
-    hdt = abi.F16_PAIRS if (args.profile and args.profile_pairs) else abi.F16
-    batch16, ins16, ms16, B16, v16 = run_batch(hdt, "fp16" if hdt == abi.F16 else "fp16 pairs",
-                                               v=args.variant)
+    hdt = abi.F16
+    if args.profile and args.profile_pairs:
+        hdt = abi.F16_PAIRS
+    elif args.profile and args.profile_u8:
+        hdt = abi.U8
+    htag = {abi.F16: "fp16", abi.F16_PAIRS: "fp16 pairs", abi.U8: "u8"}[hdt]
+    batch16, ins16, ms16, B16, v16 = run_batch(hdt, htag, v=args.variant)
     modes16 = batch16.mode_ms
     checks = {}
     t0tab, _ = abi.delay_table(surveys[0])
@@ -796,8 +802,9 @@ def main():
             log(f"no traffic profile of tile variant {v16} (profiled: {sorted(tr.get('by_variant', {}))}): "
                 "roofline.traffic left null")
     line = {
-        "metric": "dedispersed DM x samples per second (batched, FP16 storage)" if hdt == abi.F16 else
-                  "dedispersed DM x samples per second (batched, FP16 storage, channel-pair sums)",
+        "metric": {abi.F16: "dedispersed DM x samples per second (batched, FP16 storage)",
+                   abi.F16_PAIRS: "dedispersed DM x samples per second (batched, FP16 storage, channel-pair sums)",
+                   abi.U8: "dedispersed DM x samples per second (batched, U8 storage)"}[hdt],
         "value": value,
         "unit": "DM*samples/s",
         "n_gpus": 1,
@@ -808,7 +815,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "storage": "f16" if hdt == abi.F16 else "f16_pairs",
+        "storage": {abi.F16: "f16", abi.F16_PAIRS: "f16_pairs", abi.U8: "u8"}[hdt],
         "data": "synthetic (seeded Philox noise, bandpass, 20 injected pulses per beam), generated on the device",
         "config": {"workload": f"{len(beams)} uneven beams, nchans 512/1024, nsamp 2^17..2^20, "
                                "ndms 1024..4096, dmstep 0.1..0.5, one launch",
