@@ -639,7 +639,7 @@ struct PairsOf<PackedTile<MDSM_F16, MW, K2, CB, BURST>> {
 // the sum rows, so the next block's windows can arrive while it runs.
 // Table: the trial offsets (one per pair and trial, relative to the sum
 // rows), then per (DM tile, block, pair) the words {dmin, n}.
-template <int MODE, int MW, int K2, int CBP, int NS>
+template <int MODE, int MW, int K2, int CBP, int NS, int BURST = 0>
 struct PairSumTile {
   static constexpr int TT = 128 * K2;
   static constexpr int TD = 4 * MW;
@@ -915,21 +915,42 @@ struct PairSumTile {
         }
         since_fold += nc;
       }
+      auto add_word = [&](int i, int j, uint32_t w) {
+        if constexpr (IS_U8) {
+          uacc[i][j] += __builtin_bit_cast(ushort2_t, w);
+        } else {
+          const half2_t h = __builtin_bit_cast(half2_t, w);
+          facc[i][j][0] = __builtin_fmaf((float)h.x, one, facc[i][j][0]);
+          facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
+        }
+      };
 #pragma unroll
       for (int p = 0; p < CBP; ++p) {
         if (2u * (uint32_t)p < nc) {
+          if constexpr (BURST > 0) {
+            // the reads of BURST trials are issued together ahead of their
+            // adds (a scheduling fence keeps them grouped)
 #pragma unroll
-          for (int i = 0; i < MW; ++i) {
-            const uint32_t *q = at_bytes(cur, rel.get(p, i));
+            for (int i0 = 0; i0 < MW; i0 += BURST) {
+              uint32_t w[BURST][K2];
 #pragma unroll
-            for (int j = 0; j < K2; ++j) {
-              if constexpr (IS_U8) {
-                uacc[i][j] += __builtin_bit_cast(ushort2_t, q[64 * j]);
-              } else {
-                const half2_t h = __builtin_bit_cast(half2_t, q[64 * j]);
-                facc[i][j][0] = __builtin_fmaf((float)h.x, one, facc[i][j][0]);
-                facc[i][j][1] = __builtin_fmaf((float)h.y, one, facc[i][j][1]);
+              for (int g = 0; g < BURST; ++g) {
+                const uint32_t *q = at_bytes(cur, rel.get(p, i0 + g));
+#pragma unroll
+                for (int j = 0; j < K2; ++j) w[g][j] = q[64 * j];
               }
+              __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+              for (int g = 0; g < BURST; ++g)
+#pragma unroll
+                for (int j = 0; j < K2; ++j) add_word(i0 + g, j, w[g][j]);
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < MW; ++i) {
+              const uint32_t *q = at_bytes(cur, rel.get(p, i));
+#pragma unroll
+              for (int j = 0; j < K2; ++j) add_word(i, j, q[64 * j]);
             }
           }
         }
@@ -972,8 +993,8 @@ struct PairSumTile {
 
 template <typename T>
 struct IsPairSum : std::false_type {};
-template <int MODE, int MW, int K2, int CBP, int NS>
-struct IsPairSum<PairSumTile<MODE, MW, K2, CBP, NS>> : std::true_type {};
+template <int MODE, int MW, int K2, int CBP, int NS, int BURST>
+struct IsPairSum<PairSumTile<MODE, MW, K2, CBP, NS, BURST>> : std::true_type {};
 
 // Trial offsets and {dmin, n} of a pair-sum tile (layout above): entry e <
 // nrel is ((dt * nblocks + k) * 4 + w) * WORDS + idx with (p, i) =
@@ -1232,7 +1253,10 @@ template <> struct PairSumDef<16> { template <int M> using type = PairSumTile<M,
 template <> struct PairSumDef<17> { template <int M> using type = PairSumTile<M, 12, 4, 2, 3>; };
 template <> struct PairSumDef<18> { template <int M> using type = PairSumTile<M, 12, 2, 2, 3>; };
 template <> struct PairSumDef<19> { template <int M> using type = PairSumTile<M, 16, 3, 2, 4>; };
-constexpr int kNumPairSumVariants = 4;
+// 17 with the reads of 2 or 3 trials issued ahead of their adds
+template <> struct PairSumDef<20> { template <int M> using type = PairSumTile<M, 12, 4, 2, 3, 2>; };
+template <> struct PairSumDef<21> { template <int M> using type = PairSumTile<M, 12, 4, 2, 3, 3>; };
+constexpr int kNumPairSumVariants = 6;
 
 template <typename V, typename Fn>
 int pick_dtype(int dtype, Fn &&fn) {
@@ -1271,6 +1295,8 @@ int with_tile(int dtype, int variant, Fn &&fn) {
       case 17: return fn((PairSumDef<17>::type<MDSM_F16_PAIRS> *)nullptr);
       case 18: return fn((PairSumDef<18>::type<MDSM_F16_PAIRS> *)nullptr);
       case 19: return fn((PairSumDef<19>::type<MDSM_F16_PAIRS> *)nullptr);
+      case 20: return fn((PairSumDef<20>::type<MDSM_F16_PAIRS> *)nullptr);
+      case 21: return fn((PairSumDef<21>::type<MDSM_F16_PAIRS> *)nullptr);
     }
   }
   if (dtype == MDSM_U8) {
@@ -1279,6 +1305,8 @@ int with_tile(int dtype, int variant, Fn &&fn) {
       case 17: return fn((PairSumDef<17>::type<MDSM_U8> *)nullptr);
       case 18: return fn((PairSumDef<18>::type<MDSM_U8> *)nullptr);
       case 19: return fn((PairSumDef<19>::type<MDSM_U8> *)nullptr);
+      case 20: return fn((PairSumDef<20>::type<MDSM_U8> *)nullptr);
+      case 21: return fn((PairSumDef<21>::type<MDSM_U8> *)nullptr);
     }
   }
   set_error("unknown dedispersion variant %d", variant);
