@@ -142,8 +142,10 @@ int mdsm_plan_device_table(mdsm_plan plan, const int32_t **table);
 int mdsm_plan_set_variant(mdsm_plan plan, int variant);
 int mdsm_plan_get_variant(mdsm_plan plan, int *variant);
 int mdsm_tile_variants(int in_dtype, int *count);
-/* The untuned default tile shape of a storage type (a plan uses it when it
- * can stage the survey's delay spread, else the first shape that can). */
+/* The untuned default tile shape of a storage type.  A plan or batch uses
+ * it when it takes the survey; otherwise MDSM_F16_PAIRS and MDSM_U8 fall
+ * back to shapes 15 and 2 (which take every survey whose delay spread they
+ * can stage), and any type to the first shape that can. */
 int mdsm_tile_preferred(int in_dtype, int *variant);
 /* Channels a tile shape stages per LDS block for this plan's delay spread;
  * 0 when the shape cannot stage it. */

@@ -288,10 +288,13 @@ static bool variant_fits(const mdsm_plan_s *p, int v) {
 }
 
 // Untuned default: the preferred tile shape if it can stage this survey's
-// delay spread, else the first one that can.
+// delay spread, else the storage type's fallback shape, else the first one
+// that can.
 static int default_variant(const mdsm_plan_s *p) {
   const int pref = preferred_variant(p->dtype);
   if (variant_fits(p, pref)) return pref;
+  const int fb = fallback_variant(p->dtype);
+  if (variant_fits(p, fb)) return fb;
   for (int v = 1; v < (int)p->row_words.size(); ++v)
     if (variant_fits(p, v)) return v;
   return pref;

@@ -66,12 +66,14 @@ static bool batch_fits(const mdsm_batch_s *b, int v) {
 }
 
 // Untuned default: the preferred tile shape when it fits every beam, else the
-// first shape that does.
+// storage type's fallback shape, else the first shape that does.
 static int batch_variant(const mdsm_batch_s *b) {
   if (b->variant) return b->variant;
   auto fits = [&](int v) { return batch_fits(b, v); };
   const int pref = preferred_variant(b->dtype);
   if (fits(pref)) return pref;
+  const int fb = fallback_variant(b->dtype);
+  if (fits(fb)) return fb;
   for (int v = 1; v <= num_variants(b->dtype); ++v)
     if (fits(v)) return v;
   return pref;

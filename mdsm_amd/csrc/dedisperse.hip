@@ -1333,8 +1333,25 @@ int num_variants(int dtype) {
                                                        : kNumVariants;
 }
 
+// Untuned defaults, the fastest shapes measured at 1024 channels x 4096
+// trials: F16_PAIRS and U8 prefer the pair-sum shapes (20, 21), which take
+// only surveys with few delay differences per channel pair and DM tile, and
+// fall back to the fastest shape that takes every survey (15, 2).
 int preferred_variant(int dtype) {
-  return dtype == MDSM_F32 ? 8 : (storage_dtype(dtype) == MDSM_F16 ? 15 : 2);
+  switch (dtype) {
+    case MDSM_F32: return 8;
+    case MDSM_F16: return 15;
+    case MDSM_F16_PAIRS: return 20;
+    default: return 21;
+  }
+}
+
+int fallback_variant(int dtype) {
+  switch (dtype) {
+    case MDSM_F16_PAIRS: return 15;
+    case MDSM_U8: return 2;
+    default: return preferred_variant(dtype);
+  }
 }
 
 TileShape variant_shape(int dtype, int variant) {

@@ -73,6 +73,9 @@ int build_delay_table(const mdsm_survey_t *s, std::vector<int32_t> &table, uint3
 // Untuned default tile variant per storage type (the fastest measured on a
 // 1024-channel, 4096-trial beam).
 int preferred_variant(int dtype);
+// The shape an untuned plan takes when the preferred one does not take the
+// survey (F16_PAIRS, U8: the fastest shape that takes every survey).
+int fallback_variant(int dtype);
 // Channels staged per LDS block for a tile variant and row width (0: does not fit).
 uint32_t stage_channels(int dtype, int variant, uint32_t row_words);
 // Words of a variant's trial-offset table, and the kernel that fills it

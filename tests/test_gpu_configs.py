@@ -116,12 +116,13 @@ def _channel_constant_expect(dtype, C):
 
 
 @pytest.mark.parametrize("dtype,variant", [(abi.F32, None), (abi.F16, None), (abi.U8, None),
-                                           (abi.F16_PAIRS, None), (abi.F16_PAIRS, 17), (abi.U8, 17)],
-                         ids=["f32", "f16", "u8", "f16_pairs", "f16_pairsum_tile", "u8_pairsum_tile"])
+                                           (abi.F16_PAIRS, None), (abi.F16_PAIRS, 15), (abi.U8, 2)],
+                         ids=["f32", "f16", "u8", "f16_pairs", "f16_pairs_packed_tile", "u8_packed_tile"])
 def test_batch_32_beams_full_size(batch32, dtype, variant):
-    """The 32-beam batch in each storage type (untuned shape; the FP16
-    channel-pair sums and U8 also with the 12 x 4 pair-sum tile shape, which
-    takes every beam of this batch)."""
+    """The 32-beam batch in each storage type with its untuned shape (for
+    the FP16 channel-pair sums and U8 a pair-sum tile shape, which takes
+    every beam of this batch), and the FP16 channel-pair sums and U8 also
+    with the shapes that take every survey (15 and 2)."""
     beams, surveys, outs, checked = batch32
     batch = abi.Batch(surveys, dtype)
     if variant is not None:
