@@ -208,6 +208,30 @@ def test_f16_pairs_rounding_cases_all_variants(C):
         assert p.variant_channels(abi.tile_variants() + 1) > 0
 
 
+@pytest.mark.parametrize("dtype,fallback", [(abi.F16_PAIRS, 15), (abi.U8, 2)])
+def test_untuned_default_pairsum_or_fallback(dtype, fallback):
+    """Untuned plans take the pair-sum shape where it takes the survey (many
+    narrow channels) and the storage type's shape for every survey where it
+    does not (few wide channels); both equal the oracle."""
+    pref = abi.preferred_variant(dtype)
+    assert pref > abi.tile_variants()  # a pair-sum shape
+    for C, foff, want in ((512, -0.0390625, pref), (32, -0.625, fallback)):
+        s = _survey(2000, nchans=C, ndms=60, dmstep=0.25, foff=foff)
+        t = _table(s)
+        S = int(t.max())
+        p = abi.Plan(s, dtype)
+        _say(f"[untuned default] dtype {dtype} C={C} S={S} -> variant {p.variant}")
+        assert p.variant == want
+        rng = np.random.default_rng(C)
+        if dtype == abi.U8:
+            x = rng.integers(0, 256, (C, 2000 + S + 4)).astype(np.uint8)
+            ref = O.dedisperse_u8(x, t, 2000)
+        else:
+            x = O.f32_to_f16(rng.standard_normal((C, 2000 + S + 4)).astype(np.float32))
+            ref = O.dedisperse_f16_pairs(x, t, 2000)
+        assert np.array_equal(abi.dedisperse_host(s, x, dtype), ref)
+
+
 def test_u8_near_saturated_sums_all_variants():
     """Bytes of 250..255 over 1500 channels: sums up to 382500 need the
     high-part accumulator after every 256-channel window, in every variant."""
