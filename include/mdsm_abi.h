@@ -129,7 +129,7 @@ int mdsm_plan_info(mdsm_plan plan, uint32_t *maxshift, uint64_t *min_ld);
 int mdsm_plan_device_table(mdsm_plan plan, const int32_t **table);
 /* Kernel tile variant (0 = the storage type's default); see
  * mdsm_tile_variants(): 15 shapes per storage type, and for MDSM_F16_PAIRS
- * and MDSM_U8 six more (16-21) that build each channel pair's sums (FP16,
+ * and MDSM_U8 seven more (16-22) that build each channel pair's sums (FP16,
  * or exact u16) once per delay difference of its two channels; those take
  * a survey only when that
  * difference is >= 0 and spans at most 3 values (19: 4) in every DM tile of

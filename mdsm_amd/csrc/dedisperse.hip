@@ -1256,7 +1256,9 @@ template <> struct PairSumDef<19> { template <int M> using type = PairSumTile<M,
 // 17 with the reads of 2 or 3 trials issued ahead of their adds
 template <> struct PairSumDef<20> { template <int M> using type = PairSumTile<M, 12, 4, 2, 3, 2>; };
 template <> struct PairSumDef<21> { template <int M> using type = PairSumTile<M, 12, 4, 2, 3, 3>; };
-constexpr int kNumPairSumVariants = 6;
+// 10 trials x 4 pairs with grouped reads: 4 waves per SIMD for U8 (126 VGPRs)
+template <> struct PairSumDef<22> { template <int M> using type = PairSumTile<M, 10, 4, 2, 3, 2>; };
+constexpr int kNumPairSumVariants = 7;
 
 template <typename V, typename Fn>
 int pick_dtype(int dtype, Fn &&fn) {
@@ -1297,6 +1299,7 @@ int with_tile(int dtype, int variant, Fn &&fn) {
       case 19: return fn((PairSumDef<19>::type<MDSM_F16_PAIRS> *)nullptr);
       case 20: return fn((PairSumDef<20>::type<MDSM_F16_PAIRS> *)nullptr);
       case 21: return fn((PairSumDef<21>::type<MDSM_F16_PAIRS> *)nullptr);
+      case 22: return fn((PairSumDef<22>::type<MDSM_F16_PAIRS> *)nullptr);
     }
   }
   if (dtype == MDSM_U8) {
@@ -1307,6 +1310,7 @@ int with_tile(int dtype, int variant, Fn &&fn) {
       case 19: return fn((PairSumDef<19>::type<MDSM_U8> *)nullptr);
       case 20: return fn((PairSumDef<20>::type<MDSM_U8> *)nullptr);
       case 21: return fn((PairSumDef<21>::type<MDSM_U8> *)nullptr);
+      case 22: return fn((PairSumDef<22>::type<MDSM_U8> *)nullptr);
     }
   }
   set_error("unknown dedispersion variant %d", variant);
@@ -1334,7 +1338,7 @@ int num_variants(int dtype) {
 }
 
 // Untuned defaults, the fastest shapes measured at 1024 channels x 4096
-// trials: F16_PAIRS and U8 prefer the pair-sum shapes (20, 21), which take
+// trials: F16_PAIRS and U8 prefer the pair-sum shapes (20, 22), which take
 // only surveys with few delay differences per channel pair and DM tile, and
 // fall back to the fastest shape that takes every survey (15, 2).
 int preferred_variant(int dtype) {
@@ -1342,7 +1346,7 @@ int preferred_variant(int dtype) {
     case MDSM_F32: return 8;
     case MDSM_F16: return 15;
     case MDSM_F16_PAIRS: return 20;
-    default: return 21;
+    default: return 22;
   }
 }
 
