@@ -616,22 +616,22 @@ struct PairsOf<PackedTile<MDSM_F16, MW, K2, CB, BURST>> {
   using type = PackedTile<MDSM_F16_PAIRS, MW, K2, CB, BURST / 2>;
 };
 
-// ====================================================== F16 pair-sum rows
-// F16_PAIRS tiles that add each channel pair once per distinct delay
+// ================================================ pair-sum rows (F16, U8)
+// F16_PAIRS and U8 tiles that add each channel pair once per distinct delay
 // difference instead of once per trial.  For channels c0 = 2k, c1 = 2k+1
-// and trial d, the pair sum fl16(x0[t + D0(d)] + x1[t + D1(d)]) depends on
-// d only through D0(d) and delta(d) = D1(d) - D0(d), and inside one DM tile
-// delta takes a few consecutive values dmin .. dmin+n-1.  After a channel
-// block is staged (one raw window per channel), the build pass adds the
-// pair's windows once per value into sum rows S_j[s] = fl16(x0[s] +
-// x1[s + dmin + j]) (a P and a Q copy, as in the F16 layout), and each trial
-// reads one word of S_{delta(d)-dmin} at D0(d): one LDS word and two
-// v_fma_mix_f32 per four channel adds, where the F16_PAIRS layout above reads
-// two words and adds them with a v_pk_add_f16 for every trial.  The sums are
-// the same IEEE operations on the same operands, so the results are
-// identical to that layout's.  A shape takes a survey when n <= NS and
-// delta >= 0 for every DM tile and pair (pairsum_accepts); the F16_PAIRS
-// forms of the F16 shapes take every survey.
+// and trial d, the pair sum x0[t + D0(d)] + x1[t + D1(d)] (F16_PAIRS: one
+// FP16 rounding; U8: exact in 16 bits) depends on d only through D0(d) and
+// delta(d) = D1(d) - D0(d), and inside one DM tile delta takes a few
+// consecutive values dmin .. dmin+n-1.  After a channel block is staged (one
+// raw window per channel), the build pass adds the pair's windows once per
+// value into sum rows S_j[s] = x0[s] + x1[s + dmin + j] (a P and a Q copy,
+// as in the F16 layout), and each trial reads one word of S_{delta(d)-dmin}
+// at D0(d): one LDS word per four channel adds, then two v_fma_mix_f32
+// (F16_PAIRS) or one v_pk_add_u16 (U8), where the packed layouts above read
+// two words per trial.  The sums are the same operations on the same
+// operands, so the results are identical to those layouts'.  A shape takes
+// a survey when n <= NS and delta >= 0 for every DM tile and pair
+// (pairsum_accepts); the packed shapes take every survey.
 //
 // LDS: the raw windows of the block's cbr channels (wp words each, one
 // buffer: the build has consumed them before the next block is staged),
@@ -717,7 +717,7 @@ struct PairSumTile {
     }
   }
 
-  // The n sum rows of one pair over part of the window: row j is
+  // F16: the n sum rows of one pair over part of the window: row j is
   // P_j[k] = fl16(R0[k] + R1 shifted by SH = j + F samples) (two halves per
   // word, one v_pk_add_f16) and its Q copy, F = dmin & 1.  Lane l builds
   // words k0 + 4 l .. + 3 of every `step`; R0 and R1 are read once for all
