@@ -147,10 +147,12 @@ def test_packed_ragged_shapes_all_variants(shape):
 
 # (nchans, ndms, nsamp, lowdm, dmstep, foff): surveys whose channel pairs
 # differ in delay by at most three values per DM tile, so the pair-sum tile
-# shapes take them (odd channel counts, partial DM tiles and time tiles)
+# shapes take them (odd channel counts, partial DM tiles and time tiles, one
+# channel summed with zero, one trial shorter than a time tile)
 PAIRSUM_SURVEYS = [(256, 100, 1300, 0.0, 0.1, -0.078125), (255, 70, 1000, 0.0, 0.1, -0.078125),
                    (128, 64, 2100, 5.0, 0.1, -0.078125), (99, 49, 777, 0.0, 0.1, -0.078125),
-                   (512, 40, 600, 0.0, 0.25, -0.0390625), (1024, 300, 5000, 0.0, 0.2, -0.0390625)]
+                   (512, 40, 600, 0.0, 0.25, -0.0390625), (1024, 300, 5000, 0.0, 0.2, -0.0390625),
+                   (1, 5, 300, 0.0, 0.1, -0.078125), (64, 1, 100, 0.0, 0.1, -0.078125)]
 
 
 @pytest.mark.parametrize("shape", PAIRSUM_SURVEYS)
