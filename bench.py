@@ -625,7 +625,7 @@ def main():
     tl, _ = abi.delay_table(surveys[-1])
     checks["beam_last"] = spot_check(beams[-1], tl, ins16[-1][0], ins16[-1][1], hdt, outs[-1],
                                      [1, surveys[-1].ndms - 2])
-    log(f"[batch fp16] spot checks {checks}")
+    log(f"[batch {htag}] spot checks {checks}")
     det = {"candidates": 0, "clusters": 0, "pulses_found": 0, "pulses": 0}
     if not args.profile:
         tm = abi.Timer(stream)
