@@ -1181,13 +1181,30 @@ __device__ __forceinline__ void run_item(const BeamDesc *__restrict__ beams, con
             uniform(it.dt), lds);
 }
 
+// Waves per SIMD the persistent kernel is compiled for: its item loop costs
+// a few VGPRs over the flat kernel, which drops the 12 x 4 and 10 x 4
+// pair-sum tiles from 4 to 3 waves (above 128 VGPRs).  Held at 4 they spill
+// a few dwords and still run faster (32-beam batch, persistent launch: FP16
+// pair sums 1160 against 1240-1266 ms, U8 1076 against 1270-1285 ms); for
+// the 8 x 4 FP16 tile the same hold (6 waves) was slower (1795 against
+// 1705-1725 ms), so the other tiles are left to the compiler.
+template <typename Tile>
+struct PersistWaves {
+  static constexpr int value = 1;
+};
+template <int MODE, int MW, int K2, int CBP, int NS, int BURST>
+struct PersistWaves<PairSumTile<MODE, MW, K2, CBP, NS, BURST>> {
+  static constexpr int value = 4;
+};
+
 // Persistent multi-beam kernel: resident workgroups pull items from eight
 // queues, one per XCD slot (workgroup b drains queue b % 8, the XCD the
 // dispatcher puts it on), until the queue passes total_items; every wave
 // of every workgroup reaches the exit.  Beams are ordered by the host,
 // largest first.
 template <typename Tile>
-__global__ __launch_bounds__(256) void batch_kernel(const BeamDesc *__restrict__ beams,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PersistWaves<Tile>::value)))
+void batch_kernel(const BeamDesc *__restrict__ beams,
                                                     uint32_t nbeams, uint32_t total_items,
                                                     uint32_t cbr, uint32_t *__restrict__ counter) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
